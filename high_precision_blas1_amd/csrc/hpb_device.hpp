@@ -1,0 +1,141 @@
+// hpb_device.hpp -- device-side building blocks for gfx950 (wave64).
+//
+// Error-free transformations and the (s, c) accumulator merges used by every
+// compensated reduction in the library.
+//   TwoSum (Knuth) ........ source/common/ogita/gpu_reduction_ogita_impl_functions.cuh:115-123
+//   TwoProd (FMA) ......... gpu_reduction_ogita_impl_functions.cuh:89-113
+//   per-element Dot2 step . gpu_reduction_ogita_impl.cuh:498-501  (c += pi + t)
+//   pair merge ............ gpu_reduction_ogita_impl.cuh:532-534  (s = TwoSum(s,s'), c = (c+t)+c')
+// The tree shape is our own (wave64 butterfly + LDS across waves + fixed-order
+// final stage), not the reference's 1024-thread / warp-32 tree.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hpb {
+namespace dev {
+
+template <typename T> __device__ __forceinline__ T fma_rn(T a, T b, T c);
+template <> __device__ __forceinline__ double fma_rn<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <> __device__ __forceinline__ float fma_rn<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// s = fl(a + b), a + b = s + t exactly (no overflow).
+template <typename T>
+__device__ __forceinline__ T two_sum(T a, T b, T& t)
+{
+    T s = a + b;
+    T bs = s - a;
+    T as = s - bs;
+    t = (b - bs) + (a - as);
+    return s;
+}
+
+// p = fl(a b), a b = p + e exactly (no underflow below the FMA's reach).
+template <typename T>
+__device__ __forceinline__ T two_prod(T a, T b, T& e)
+{
+    T p = a * b;
+    e = fma_rn<T>(a, b, -p);
+    return p;
+}
+
+// Compensated accumulator state of one lane.
+template <typename T>
+struct sc_t {
+    T s, c;
+};
+
+// ORO Dot2 step: (s, c) += x*y.
+template <typename T>
+__device__ __forceinline__ void dot2_step(T& s, T& c, T x, T y)
+{
+    T pi, t;
+    T p = two_prod<T>(x, y, pi);
+    s = two_sum<T>(s, p, t);
+    c = c + (pi + t);
+}
+
+// ORO Sum2 step: (s, c) += v.
+template <typename T>
+__device__ __forceinline__ void sum2_step(T& s, T& c, T v)
+{
+    T t;
+    s = two_sum<T>(s, v, t);
+    c = c + t;
+}
+
+// Merge (s2, c2) into (s, c): Sum2 over the s parts, plain sum of the c parts.
+template <typename T>
+__device__ __forceinline__ void merge(T& s, T& c, T s2, T c2)
+{
+    T t;
+    s = two_sum<T>(s, s2, t);
+    c = (c + t) + c2;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int m)
+{
+    return __shfl_xor(v, m, 64);
+}
+
+// Butterfly over the 64 lanes of a wave; every lane ends with the wave's total.
+// Level order: lane distance 32, 16, ..., 1; each lane merges its partner into
+// its own value.
+template <typename T, bool COMP>
+__device__ __forceinline__ void wave_reduce(T& s, T& c)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        T s2 = shfl_xor<T>(s, m);
+        if (COMP) {
+            T c2 = shfl_xor<T>(c, m);
+            merge<T>(s, c, s2, c2);
+        } else {
+            s = s + s2;
+        }
+    }
+}
+
+// Block reduction: wave butterflies, then lane 0 of wave 0 merges the wave
+// totals in wave order.  Result valid in thread 0 only.  Uses 2*NW words of LDS.
+template <typename T, bool COMP, int BLOCK>
+__device__ __forceinline__ void block_reduce(T& s, T& c)
+{
+    constexpr int NW = BLOCK / 64;
+    __shared__ T lds_s[NW];
+    __shared__ T lds_c[NW];
+    wave_reduce<T, COMP>(s, c);
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    if (NW > 1) {
+        if (lane == 0) {
+            lds_s[wid] = s;
+            lds_c[wid] = c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < NW; ++w) {
+                if (COMP) merge<T>(s, c, lds_s[w], lds_c[w]);
+                else s = s + lds_s[w];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// 16-byte vector type per element type.
+template <typename T> struct vec16;
+template <> struct vec16<double> { using type = double2; static constexpr int W = 2; };
+template <> struct vec16<float> { using type = float4; static constexpr int W = 4; };
+
+template <typename T>
+__device__ __forceinline__ T vget(const typename vec16<T>::type& v, int i);
+template <> __device__ __forceinline__ double vget<double>(const double2& v, int i) { return i == 0 ? v.x : v.y; }
+template <> __device__ __forceinline__ float vget<float>(const float4& v, int i)
+{
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+}  // namespace dev
+}  // namespace hpb

@@ -1,0 +1,65 @@
+// hpb_internal.hpp -- library-private handle, status plumbing and launch tables.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/hpb.h"
+
+struct hpb_tuning_t {
+    int block;      // threads per block (multiple of 64)
+    int unroll;     // 16-byte vectors per thread per tile
+    int grid_max;   // cap on the number of blocks (grid-stride beyond it)
+};
+
+struct hpb_handle_s {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    int num_cu = 0;
+    // Reduction workspace: per-block (s, c) partials of the current launch, the
+    // last-block ticket and a device result area.  Ops on one handle run in
+    // stream order, so one set suffices.
+    void* part = nullptr;
+    size_t part_bytes = 0;
+    unsigned* ticket = nullptr;     // HPB_TICKETS counters, each returns to 0 after use
+    void* dres = nullptr;           // device results, HPB_DRES_BYTES
+    void* hres = nullptr;           // pinned host mirror of dres
+    int* dflag = nullptr;           // validity flag
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hpb_tuning_t tune_red[2];       // [0] f64, [1] f32 reductions
+    hpb_tuning_t tune_map[2];       // elementwise kernels
+};
+
+#define HPB_TICKETS 4
+#define HPB_DRES_BYTES (64 * 4 * 8 * 4)
+#define HPB_MAX_GRID 4096
+#define HPB_MAX_MULTIDOT 64
+
+namespace hpb {
+
+void set_error(const char* file, int line, const std::string& msg);
+int hip_status(hipError_t e, const char* file, int line, const char* what);
+
+}  // namespace hpb
+
+#define HPB_HIP(call)                                                            \
+    do {                                                                         \
+        hipError_t _e = (call);                                                  \
+        if (_e != hipSuccess) return hpb::hip_status(_e, __FILE__, __LINE__, #call); \
+    } while (0)
+
+#define HPB_REQUIRE(cond, code, msg)                                             \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            hpb::set_error(__FILE__, __LINE__, msg);                             \
+            return (code);                                                       \
+        }                                                                        \
+    } while (0)
+
+#define HPB_LAUNCH_CHECK()                                                       \
+    do {                                                                         \
+        hipError_t _e = hipGetLastError();                                       \
+        if (_e != hipSuccess) return hpb::hip_status(_e, __FILE__, __LINE__, "kernel launch"); \
+    } while (0)

@@ -1,0 +1,346 @@
+// hpb_reduce.hip -- single-launch compensated and plain reductions for gfx950.
+//
+// Replaces gpu_reduction_ogita (source/common/ogita/gpu_reduction_ogita_impl.cuh:35-676,
+// 1169-1282) and gpu_reduction (source/common/gpu_reduction_impl.cuh:335-841,
+// 1121-1244).  Design:
+//   * one launch per reduction: a bounded grid of G <= grid_max blocks streams the
+//     vectors with 16-byte loads (double2 / float4), UNROLL loads per input per
+//     thread in flight, grid-stride over tiles of BLOCK*UNROLL vectors;
+//   * every lane keeps one (s, c) Ogita-Rump-Oishi accumulator; nothing is written
+//     per element (the reference zeroes and re-reads an N-sized error array);
+//   * wave64 butterfly merge with TwoSum, LDS across the waves of a block;
+//   * per-block (s, c) partials go to a handle-owned workspace of 2G words; the
+//     block that takes the last ticket merges them in block-index order, so the
+//     result does not depend on which block finishes last (reproducible); no
+//     floating-point atomics, no in-place multi-pass (the reference's passes read
+//     and write the same array from different blocks, impl.cuh:1247-1253).
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "hpb_device.hpp"
+#include "hpb_internal.hpp"
+
+namespace hpb {
+
+enum red_op { R_DOT2 = 0, R_DOT, R_SUM2, R_SUM, R_ASUM2, R_ASUM, R_NRM2C, R_NRM2 };
+
+template <int OP> struct red_traits {
+    static constexpr bool binary = (OP == R_DOT2 || OP == R_DOT);
+    static constexpr bool comp = (OP == R_DOT2 || OP == R_SUM2 || OP == R_ASUM2 || OP == R_NRM2C);
+    static constexpr bool nrm = (OP == R_NRM2C || OP == R_NRM2);
+};
+
+template <typename T> __device__ __forceinline__ T absv(T v);
+template <> __device__ __forceinline__ double absv<double>(double v) { return __builtin_fabs(v); }
+template <> __device__ __forceinline__ float absv<float>(float v) { return __builtin_fabsf(v); }
+
+template <typename T> __device__ __forceinline__ T sqrtv(T v);
+template <> __device__ __forceinline__ double sqrtv<double>(double v) { return __builtin_sqrt(v); }
+template <> __device__ __forceinline__ float sqrtv<float>(float v) { return __builtin_sqrtf(v); }
+
+template <typename T, int OP>
+__device__ __forceinline__ void red_acc(T& s, T& c, T xv, T yv)
+{
+    if constexpr (OP == R_DOT2) dev::dot2_step<T>(s, c, xv, yv);
+    else if constexpr (OP == R_DOT) s = dev::fma_rn<T>(xv, yv, s);
+    else if constexpr (OP == R_SUM2) dev::sum2_step<T>(s, c, xv);
+    else if constexpr (OP == R_SUM) s = s + xv;
+    else if constexpr (OP == R_ASUM2) dev::sum2_step<T>(s, c, absv<T>(xv));
+    else if constexpr (OP == R_ASUM) s = s + absv<T>(xv);
+    else if constexpr (OP == R_NRM2C) dev::dot2_step<T>(s, c, xv, xv);
+    else s = dev::fma_rn<T>(xv, xv, s);
+}
+
+template <typename T> struct bits_of { using type = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned>; };
+
+template <typename T>
+__device__ __forceinline__ void st_agent(T* p, T v)
+{
+    using U = typename bits_of<T>::type;
+    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T* p)
+{
+    using U = typename bits_of<T>::type;
+    U u = __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(T, u);
+}
+
+template <typename T, typename V>
+__device__ __forceinline__ T lane_of(const V& v, int w)
+{
+    if constexpr (std::is_same<T, V>::value) return v;
+    else return dev::vget<T>(v, w);
+}
+
+template <typename T, int OP>
+__device__ __forceinline__ void write_out(T* out, T s, T c)
+{
+    T r = s + c;
+    out[0] = s;
+    out[1] = c;
+    out[2] = r;
+    out[3] = red_traits<OP>::nrm ? sqrtv<T>(r) : r;
+}
+
+// Fixed-order final stage, run by the block holding the last ticket: merges the
+// G partials in index order (thread t takes t, t+BLOCK, ...; then block_reduce).
+template <typename T, bool COMP, int BLOCK>
+__device__ __forceinline__ void final_merge(const T* part, unsigned G, T& fs, T& fc)
+{
+    fs = T(0);
+    fc = T(0);
+    for (unsigned i = threadIdx.x; i < G; i += BLOCK) {
+        T ps = ld_agent<T>(&part[2 * i]);
+        T pc = ld_agent<T>(&part[2 * i + 1]);
+        if (COMP) dev::merge<T>(fs, fc, ps, pc);
+        else fs = fs + ps;
+    }
+    dev::block_reduce<T, COMP, BLOCK>(fs, fc);
+}
+
+// Returns true in every thread of the block that took the last ticket.
+__device__ __forceinline__ bool take_ticket(unsigned* ticket, unsigned G)
+{
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == G - 1);
+    }
+    __syncthreads();
+    bool last = am_last != 0;
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return last;
+}
+
+template <typename T, int OP, int BLOCK, int UNROLL, bool VEC>
+__global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict__ x, const T* __restrict__ y,
+                                                  T* __restrict__ part, unsigned* __restrict__ ticket,
+                                                  T* __restrict__ out)
+{
+    using tr = red_traits<OP>;
+    using V = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
+    constexpr int W = VEC ? dev::vec16<T>::W : 1;
+    constexpr size_t TILE = (size_t)BLOCK * UNROLL;
+
+    const unsigned G = gridDim.x;
+    const unsigned b = blockIdx.x;
+    const unsigned tid = threadIdx.x;
+    const size_t nv = n / W;
+    const size_t nfull = nv / TILE;
+    const V* __restrict__ xv = reinterpret_cast<const V*>(x);
+    const V* __restrict__ yv = reinterpret_cast<const V*>(y);
+
+    T s = T(0), c = T(0);
+    for (size_t tile = b; tile < nfull; tile += G) {
+        const size_t base = tile * TILE + tid;
+        V xa[UNROLL], ya[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            xa[u] = xv[base + (size_t)u * BLOCK];
+            if constexpr (tr::binary) ya[u] = yv[base + (size_t)u * BLOCK];
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                if constexpr (tr::binary) red_acc<T, OP>(s, c, lane_of<T>(xa[u], w), lane_of<T>(ya[u], w));
+                else red_acc<T, OP>(s, c, lane_of<T>(xa[u], w), T(0));
+            }
+        }
+    }
+    // The partial tile goes to the block whose turn it is in the grid-stride
+    // sequence; the < W trailing scalars of the vector path follow it.
+    if (b == (unsigned)(nfull % G)) {
+        const size_t base = nfull * TILE + tid;
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const size_t vi = base + (size_t)u * BLOCK;
+            if (vi < nv) {
+                V a = xv[vi];
+                V bb = a;
+                if constexpr (tr::binary) bb = yv[vi];
+#pragma unroll
+                for (int w = 0; w < W; ++w) red_acc<T, OP>(s, c, lane_of<T>(a, w), lane_of<T>(bb, w));
+            }
+        }
+        if constexpr (VEC) {
+            const size_t r0 = nv * W;
+            if (tid < n - r0) red_acc<T, OP>(s, c, x[r0 + tid], tr::binary ? y[r0 + tid] : T(0));
+        }
+    }
+
+    dev::block_reduce<T, tr::comp, BLOCK>(s, c);
+    if (G == 1) {
+        if (tid == 0) write_out<T, OP>(out, s, c);
+        return;
+    }
+    if (tid == 0) {
+        st_agent<T>(&part[2 * b], s);
+        st_agent<T>(&part[2 * b + 1], c);
+    }
+    if (!take_ticket(ticket, G)) return;
+    T fs, fc;
+    final_merge<T, tr::comp, BLOCK>(part, G, fs, fc);
+    if (tid == 0) {
+        write_out<T, OP>(out, fs, fc);
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+template <typename T> struct dtype_id;
+template <> struct dtype_id<double> { static constexpr int v = 0; };
+template <> struct dtype_id<float> { static constexpr int v = 1; };
+
+struct red_shape {
+    int block, unroll, grid, vecw;
+};
+
+template <typename T>
+static red_shape reduce_shape(const hpb_handle_s* h, size_t n, bool aligned)
+{
+    const hpb_tuning_t& t = h->tune_red[dtype_id<T>::v];
+    red_shape r;
+    r.block = t.block;
+    r.unroll = t.unroll;
+    r.vecw = aligned ? dev::vec16<T>::W : 1;
+    const size_t nv = n / (size_t)r.vecw;
+    const size_t tile = (size_t)t.block * (size_t)t.unroll;
+    size_t ntiles = (nv + tile - 1) / tile;
+    if (ntiles < 1) ntiles = 1;
+    r.grid = (int)(ntiles < (size_t)t.grid_max ? ntiles : (size_t)t.grid_max);
+    return r;
+}
+
+template <typename T, int OP, int BLOCK, int UNROLL>
+static void launch_bu(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
+                      unsigned* ticket, T* out)
+{
+    if (sh.vecw > 1)
+        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y, part,
+                           ticket, out);
+    else
+        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, false>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y, part,
+                           ticket, out);
+}
+
+template <typename T, int OP>
+static void launch_red(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
+                       unsigned* ticket, T* out)
+{
+    switch (sh.block * 16 + sh.unroll) {
+    case 256 * 16 + 1: launch_bu<T, OP, 256, 1>(sh, st, n, x, y, part, ticket, out); break;
+    case 256 * 16 + 2: launch_bu<T, OP, 256, 2>(sh, st, n, x, y, part, ticket, out); break;
+    case 256 * 16 + 8: launch_bu<T, OP, 256, 8>(sh, st, n, x, y, part, ticket, out); break;
+    case 512 * 16 + 1: launch_bu<T, OP, 512, 1>(sh, st, n, x, y, part, ticket, out); break;
+    case 512 * 16 + 2: launch_bu<T, OP, 512, 2>(sh, st, n, x, y, part, ticket, out); break;
+    case 512 * 16 + 4: launch_bu<T, OP, 512, 4>(sh, st, n, x, y, part, ticket, out); break;
+    case 512 * 16 + 8: launch_bu<T, OP, 512, 8>(sh, st, n, x, y, part, ticket, out); break;
+    default: launch_bu<T, OP, 256, 4>(sh, st, n, x, y, part, ticket, out); break;
+    }
+}
+
+static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// dev_out != nullptr: asynchronous, result left in device memory.
+// otherwise: result copied to host_out (4 values), stream synchronised.
+template <typename T, int OP>
+static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev_out, T* host_out)
+{
+    HPB_REQUIRE(h, HPB_ERR_INVALID, "reduction: null handle");
+    HPB_REQUIRE(n == 0 || x, HPB_ERR_INVALID, "reduction: null x");
+    HPB_REQUIRE(!red_traits<OP>::binary || n == 0 || y, HPB_ERR_INVALID, "reduction: null y");
+    HPB_REQUIRE(dev_out || host_out, HPB_ERR_INVALID, "reduction: null output");
+    const bool al = aligned16(x) && (!red_traits<OP>::binary || aligned16(y));
+    red_shape sh = reduce_shape<T>(h, n, al);
+    T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
+    launch_red<T, OP>(sh, h->stream, n, x, y, reinterpret_cast<T*>(h->part), h->ticket, out);
+    HPB_LAUNCH_CHECK();
+    if (!dev_out) {
+        HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+        HPB_HIP(hipStreamSynchronize(h->stream));
+        const T* r = reinterpret_cast<const T*>(h->hres);
+        for (int i = 0; i < 4; ++i) host_out[i] = r[i];
+    }
+    return HPB_OK;
+}
+
+}  // namespace hpb
+
+using namespace hpb;
+
+extern "C" {
+
+int hpb_reduce_config(hpb_handle_t h, int dtype, size_t n, int aligned, int* cfg4)
+{
+    HPB_REQUIRE(h && cfg4, HPB_ERR_INVALID, "hpb_reduce_config: null argument");
+    HPB_REQUIRE(dtype == 0 || dtype == 1, HPB_ERR_INVALID, "hpb_reduce_config: dtype must be 0 or 1");
+    red_shape s = dtype == 0 ? reduce_shape<double>(h, n, aligned != 0) : reduce_shape<float>(h, n, aligned != 0);
+    cfg4[0] = s.block;
+    cfg4[1] = s.unroll;
+    cfg4[2] = s.grid;
+    cfg4[3] = s.vecw;
+    return HPB_OK;
+}
+
+#define HPB_RED_BIN(NAME, OP)                                                                        \
+    int hpb_##NAME##_f64(hpb_handle_t h, size_t n, const double* x, const double* y, double* out4)  \
+    {                                                                                                \
+        return reduce_entry<double, OP>(h, n, x, y, nullptr, out4);                                  \
+    }                                                                                                \
+    int hpb_##NAME##_f32(hpb_handle_t h, size_t n, const float* x, const float* y, float* out4)     \
+    {                                                                                                \
+        return reduce_entry<float, OP>(h, n, x, y, nullptr, out4);                                   \
+    }                                                                                                \
+    int hpb_##NAME##_f64_async(hpb_handle_t h, size_t n, const double* x, const double* y, double* d) \
+    {                                                                                                \
+        HPB_REQUIRE(d, HPB_ERR_INVALID, "async reduction: null device output");                    \
+        return reduce_entry<double, OP>(h, n, x, y, d, nullptr);                                     \
+    }                                                                                                \
+    int hpb_##NAME##_f32_async(hpb_handle_t h, size_t n, const float* x, const float* y, float* d)  \
+    {                                                                                                \
+        HPB_REQUIRE(d, HPB_ERR_INVALID, "async reduction: null device output");                    \
+        return reduce_entry<float, OP>(h, n, x, y, d, nullptr);                                      \
+    }
+
+#define HPB_RED_UN(NAME, OP)                                                                         \
+    int hpb_##NAME##_f64(hpb_handle_t h, size_t n, const double* x, double* out4)                  \
+    {                                                                                                \
+        return reduce_entry<double, OP>(h, n, x, nullptr, nullptr, out4);                            \
+    }                                                                                                \
+    int hpb_##NAME##_f32(hpb_handle_t h, size_t n, const float* x, float* out4)                    \
+    {                                                                                                \
+        return reduce_entry<float, OP>(h, n, x, nullptr, nullptr, out4);                             \
+    }
+
+#define HPB_RED_UN_ASYNC(NAME, OP)                                                                   \
+    int hpb_##NAME##_f64_async(hpb_handle_t h, size_t n, const double* x, double* d)               \
+    {                                                                                                \
+        HPB_REQUIRE(d, HPB_ERR_INVALID, "async reduction: null device output");                    \
+        return reduce_entry<double, OP>(h, n, x, nullptr, d, nullptr);                               \
+    }                                                                                                \
+    int hpb_##NAME##_f32_async(hpb_handle_t h, size_t n, const float* x, float* d)                 \
+    {                                                                                                \
+        HPB_REQUIRE(d, HPB_ERR_INVALID, "async reduction: null device output");                    \
+        return reduce_entry<float, OP>(h, n, x, nullptr, d, nullptr);                                \
+    }
+
+HPB_RED_BIN(dot2, R_DOT2)
+HPB_RED_BIN(dot, R_DOT)
+HPB_RED_UN(sum2, R_SUM2)
+HPB_RED_UN(sum, R_SUM)
+HPB_RED_UN(asum2, R_ASUM2)
+HPB_RED_UN(asum, R_ASUM)
+HPB_RED_UN(nrm2c, R_NRM2C)
+HPB_RED_UN(nrm2, R_NRM2)
+HPB_RED_UN_ASYNC(nrm2c, R_NRM2C)
+HPB_RED_UN_ASYNC(nrm2, R_NRM2)
+
+}  // extern "C"

@@ -1,0 +1,213 @@
+/*
+ * hpb.h -- C ABI of the MI355X (gfx950) compensated BLAS1 + Krylov library.
+ *
+ * The drop-in boundary for the compensated BLAS1 hot path of
+ * evstigneevnm/high_precision_blas1.  The reference exposes that path as C++
+ * templates (VectorOperations concept, gpu_reduction_ogita, csr::gpu_matrix,
+ * the lin_solvers); this header is the flat layer those templates sit on here.
+ * Every entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Vector arguments are DEVICE pointers (hipMalloc / hpb_malloc memory);
+ *     sizes are element counts (size_t).  Arguments named out_* are HOST
+ *     pointers unless the name says dev.
+ *   - Every call returns int: 0 = success, > 0 a hipError_t value, < 0 an
+ *     HPB_ERR_* code.  hpb_last_error() gives the message of the last failure
+ *     on the calling thread (file:line like the reference's CUDA_SAFE_CALL,
+ *     utils/cuda_safe_call.h:143-147).
+ *   - A handle owns one HIP stream; all work of the handle runs on it in order.
+ *     Calls returning host scalars synchronise that stream (the reference's
+ *     reductions are host-synchronous too, gpu_reduction_ogita_impl.cuh:1273-1274);
+ *     *_async calls write results to device memory and do not synchronise.
+ *   - A handle is not thread-safe; use one handle per host thread / stream.
+ *   - Compensated reductions report out[4] = { s, c, fl(s + c), sqrt(fl(s + c)) }:
+ *     the unrounded Ogita-Rump-Oishi pair, the rounded result and (for
+ *     nrm2-type use) its square root.  Plain reductions report c = 0.
+ *   - Reductions are deterministic: for a given (n, pointer alignment, tuning)
+ *     the summation order is fixed; no floating-point atomics are used.
+ */
+#ifndef HPB_H
+#define HPB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPB_OK 0
+#define HPB_ERR_INVALID (-1)     /* bad argument (null pointer, size, range) */
+#define HPB_ERR_UNSUPPORTED (-2) /* valid request this build does not support */
+#define HPB_ERR_NOT_CONVERGED (-3)
+#define HPB_ERR_BREAKDOWN (-4)
+
+typedef struct hpb_handle_s* hpb_handle_t;
+
+/* ---- status ------------------------------------------------------------ */
+/* utils/cuda_safe_call.h:143-147 (error text with file:line) */
+const char* hpb_last_error(void);
+const char* hpb_version(void);
+
+/* ---- handle ------------------------------------------------------------ */
+/* Replaces the per-object state of gpu_vector_operations (gpu_vector_operations.h:98-127)
+ * and gpu_reduction_ogita's N-sized workspaces (gpu_reduction_ogita.h:44-52):
+ * one handle owns an O(grid) workspace.  stream may be NULL (a new stream is
+ * created and owned) or an existing hipStream_t. */
+int hpb_create(hpb_handle_t* h, void* stream);
+int hpb_destroy(hpb_handle_t h);
+int hpb_get_stream(hpb_handle_t h, void** stream);
+int hpb_synchronize(hpb_handle_t h);
+int hpb_device_info(hpb_handle_t h, int* num_cu, int* device);
+
+/* ---- memory (utils/cuda_support.h:111-209: device_allocate,
+ *      host_2_device_cpy, device_2_host_cpy; device_deallocate) ------------ */
+int hpb_malloc(void** dev_ptr, size_t bytes);
+int hpb_free(void* dev_ptr);
+int hpb_memcpy_h2d(hpb_handle_t h, void* dev_dst, const void* host_src, size_t bytes);
+int hpb_memcpy_d2h(hpb_handle_t h, void* host_dst, const void* dev_src, size_t bytes);
+int hpb_memcpy_d2d(hpb_handle_t h, void* dev_dst, const void* dev_src, size_t bytes);
+
+/* ---- tuning / introspection -------------------------------------------- */
+/* kind: 0 reductions, 1 elementwise; dtype: 0 f64, 1 f32.  The autotuner
+ * (high_precision_blas1_amd/autotune.py) writes these; BLOCK_SIZE=1024 is a fixed
+ * template parameter in the reference (gpu_reduction_ogita.h:36). */
+int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max);
+int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max);
+/* Launch shape a reduction of n elements would use: cfg = {block, unroll, grid,
+ * vector width (1 if the pointers are not 16-byte aligned)}. */
+int hpb_reduce_config(hpb_handle_t h, int dtype, size_t n, int aligned16, int* cfg4);
+
+/* ---- timing (HIP events on the handle's stream) ------------------------- */
+int hpb_timer_start(hpb_handle_t h);
+int hpb_timer_stop(hpb_handle_t h, float* out_ms);
+
+/* ---- compensated (Ogita-Rump-Oishi) reductions -------------------------- */
+/* Dot2: gpu_reduction_ogita::dot (gpu_reduction_ogita.h:85-89,
+ *   gpu_reduction_ogita_impl.cuh:471-676,1227-1282) and
+ *   gpu_vector_operations::scalar_prod in high precision (gpu_vector_operations.h:257-269). */
+int hpb_dot2_f64(hpb_handle_t h, size_t n, const double* x, const double* y, double* out4);
+int hpb_dot2_f32(hpb_handle_t h, size_t n, const float* x, const float* y, float* out4);
+int hpb_dot2_f64_async(hpb_handle_t h, size_t n, const double* x, const double* y, double* dev_out4);
+int hpb_dot2_f32_async(hpb_handle_t h, size_t n, const float* x, const float* y, float* dev_out4);
+/* Sum2: gpu_reduction_ogita::sum (gpu_reduction_ogita.h:74-78, impl.cuh:248-468,1169-1223). */
+int hpb_sum2_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
+int hpb_sum2_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
+/* Compensated sum of |x_i|: gpu_reduction_ogita::asum (gpu_reduction_ogita.h:79-84).
+ * Computes the true sum of absolute values (the reference's real-T kernel sums x). */
+int hpb_asum2_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
+int hpb_asum2_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
+/* Compensated 2-norm: gpu_reduction_ogita::norm = sqrt(Dot2(x,x)) (gpu_reduction_ogita.h:91-96),
+ * reading x once; out4[3] = sqrt(fl(s + c)). */
+int hpb_nrm2c_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
+int hpb_nrm2c_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
+int hpb_nrm2c_f64_async(hpb_handle_t h, size_t n, const double* x, double* dev_out4);
+int hpb_nrm2c_f32_async(hpb_handle_t h, size_t n, const float* x, float* dev_out4);
+
+/* ---- plain (uncompensated) reductions ----------------------------------- */
+/* gpu_reduction::{dot,sum,asum} (gpu_reduction.h:71-90) and the cuBLAS
+ * standard-precision calls of gpu_vector_operations (cublas_wrap.h:525-591).
+ * dot accumulates s = fma(x_i, y_i, s). */
+int hpb_dot_f64(hpb_handle_t h, size_t n, const double* x, const double* y, double* out4);
+int hpb_dot_f32(hpb_handle_t h, size_t n, const float* x, const float* y, float* out4);
+int hpb_dot_f64_async(hpb_handle_t h, size_t n, const double* x, const double* y, double* dev_out4);
+int hpb_dot_f32_async(hpb_handle_t h, size_t n, const float* x, const float* y, float* dev_out4);
+int hpb_sum_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
+int hpb_sum_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
+int hpb_asum_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
+int hpb_asum_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
+int hpb_nrm2_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
+int hpb_nrm2_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
+int hpb_nrm2_f64_async(hpb_handle_t h, size_t n, const double* x, double* dev_out4);
+int hpb_nrm2_f32_async(hpb_handle_t h, size_t n, const float* x, float* dev_out4);
+
+/* ---- elementwise (gpu_vector_operations_impl.cuh:226-528) ---------------- */
+/* Each *_f64 has an *_f32 twin with float scalars and pointers. */
+/* assign_scalar: x := a                                  (:226-241) */
+int hpb_fill_f64(hpb_handle_t h, size_t n, double a, double* x);
+int hpb_fill_f32(hpb_handle_t h, size_t n, float a, float* x);
+/* add_mul_scalar: x := mul_x*x + scalar (one fma)          (:243-256) */
+int hpb_add_mul_scalar_f64(hpb_handle_t h, size_t n, double scalar, double mul_x, double* x);
+int hpb_add_mul_scalar_f32(hpb_handle_t h, size_t n, float scalar, float mul_x, float* x);
+/* scale: x := a*x (cublasDscal, gpu_vector_operations.h:433-436) */
+int hpb_scal_f64(hpb_handle_t h, size_t n, double a, double* x);
+int hpb_scal_f32(hpb_handle_t h, size_t n, float a, float* x);
+/* assign: y := x (cublasDcopy, gpu_vector_operations.h:442-446) */
+int hpb_copy_f64(hpb_handle_t h, size_t n, const double* x, double* y);
+int hpb_copy_f32(hpb_handle_t h, size_t n, const float* x, float* y);
+/* swap: x <-> y (cublasDswap, gpu_vector_operations.h:448-451) */
+int hpb_swap_f64(hpb_handle_t h, size_t n, double* x, double* y);
+int hpb_swap_f32(hpb_handle_t h, size_t n, float* x, float* y);
+/* make_abs_copy / make_abs: y := |x| (x == y allowed)     (:305-332) */
+int hpb_abs_f64(hpb_handle_t h, size_t n, const double* x, double* y);
+int hpb_abs_f32(hpb_handle_t h, size_t n, const float* x, float* y);
+/* assign_mul(a,x,y): y := a*x                              (:258-272) */
+int hpb_assign_mul_f64(hpb_handle_t h, size_t n, double a, const double* x, double* y);
+int hpb_assign_mul_f32(hpb_handle_t h, size_t n, float a, const float* x, float* y);
+/* add_mul(a,x,y): y := y + a*x, one fma (cublasDaxpy, gpu_vector_operations.h:459-462) */
+int hpb_axpy_f64(hpb_handle_t h, size_t n, double a, const double* x, double* y);
+int hpb_axpy_f32(hpb_handle_t h, size_t n, float a, const float* x, float* y);
+/* assign_mul(a,x,b,y,z): z := a*x + b*y = fma(a, x, b*y)   (:334-349);
+ * add_mul(a,x,b,y) is the case z == y                     (:352-365) */
+int hpb_axpby_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y, double* z);
+int hpb_axpby_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y, float* z);
+/* add_mul(a,x,b,y,c,z): z := a*x + b*y + c*z = fma(c, z, fma(a, x, b*y))  (:368-383) */
+int hpb_axpbypcz_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y,
+                     double c, double* z);
+int hpb_axpbypcz_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y,
+                     float c, float* z);
+/* add_mul(a,x,b,y,c,w,d,z): z := a*x + b*y + c*w + d*z     (:451-466) */
+int hpb_axpbypcwpdz_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y,
+                        double c, const double* w, double d, double* z);
+int hpb_axpbypcwpdz_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y,
+                        float c, const float* w, float d, float* z);
+/* assign_mul(a,x,b,y,c,v,d,w,z): z := a*x + b*y + c*v + d*w (:468-484) */
+int hpb_assign_mul4_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y,
+                        double c, const double* v, double d, const double* w, double* z);
+int hpb_assign_mul4_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y,
+                        float c, const float* v, float d, const float* w, float* z);
+/* mul_pointwise(a,x,b,y,z): z := (a*x)*(b*y); mul_pointwise(x,b,y): x := x*(b*y)  (:386-416) */
+int hpb_mul_pointwise_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y, double* z);
+int hpb_mul_pointwise_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y, float* z);
+int hpb_mul_pointwise_inplace_f64(hpb_handle_t h, size_t n, double* x, double b, const double* y);
+int hpb_mul_pointwise_inplace_f32(hpb_handle_t h, size_t n, float* x, float b, const float* y);
+/* div_pointwise(a,x,b,y,z): z := (a*x)/(b*y); div_pointwise(x,b,y): x := x/(b*y)  (:419-449) */
+int hpb_div_pointwise_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y, double* z);
+int hpb_div_pointwise_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y, float* z);
+int hpb_div_pointwise_inplace_f64(hpb_handle_t h, size_t n, double* x, double b, const double* y);
+int hpb_div_pointwise_inplace_f32(hpb_handle_t h, size_t n, float* x, float b, const float* y);
+
+/* ---- compensated elementwise (new: no reference counterpart; the nearest is
+ *      add_mul, gpu_vector_operations.h:459-470, impl.cuh:352-383) ---------- */
+/* z := a*x + b*y evaluated error-free (TwoProd, TwoProd, TwoSum) and rounded
+ * once at the end: faithful, and correctly rounded unless the two products
+ * nearly cancel.  z == y or z == x allowed. */
+int hpb_axpbyc_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y, double* z);
+int hpb_axpbyc_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y, float* z);
+
+/* ---- misc vector ops ----------------------------------------------------- */
+/* check_is_valid_number (impl.cuh:151-224): out_valid = 1 iff all x_i finite. */
+int hpb_valid_f64(hpb_handle_t h, size_t n, const double* x, int* out_valid);
+int hpb_valid_f32(hpb_handle_t h, size_t n, const float* x, int* out_valid);
+/* set_value_at_point / get_value_at_point (impl.cuh:486-528) */
+int hpb_set_value_f64(hpb_handle_t h, size_t n, size_t at, double val, double* x);
+int hpb_set_value_f32(hpb_handle_t h, size_t n, size_t at, float val, float* x);
+int hpb_get_value_f64(hpb_handle_t h, size_t n, size_t at, const double* x, double* out_val);
+int hpb_get_value_f32(hpb_handle_t h, size_t n, size_t at, const float* x, float* out_val);
+/* assign_random(x, a, b) (gpu_vector_operations.h:496-523): x_i = a + (b-a) u_i,
+ * u_i in (0, 1] from Philox4x32-10 with key = seed, counter = (i, stream_id);
+ * seeded and reproducible (the reference seeds cuRAND from std::random_device). */
+int hpb_random_uniform_f64(hpb_handle_t h, size_t n, uint64_t seed, uint32_t stream_id,
+                           double a, double b, double* x);
+int hpb_random_uniform_f32(hpb_handle_t h, size_t n, uint64_t seed, uint32_t stream_id,
+                           float a, float b, float* x);
+/* Element conversions (generate_vector_pair_helper_impl.cuh:110-120 and the
+ * FP32/FP64 boundary of the refined solver). */
+int hpb_convert_f32_to_f64(hpb_handle_t h, size_t n, const float* x, double* y);
+int hpb_convert_f64_to_f32(hpb_handle_t h, size_t n, const double* x, float* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPB_H */
