@@ -57,14 +57,15 @@ class Handle:
         call("hpb_device_info", self._h, ctypes.byref(n), None)
         return n.value
 
-    def set_tuning(self, kind, dtype, block, unroll, grid_max):
-        call("hpb_set_tuning", self._h, kind, 0 if np.dtype(dtype) == np.float64 else 1, block, unroll, grid_max)
+    def set_tuning(self, kind, dtype, block, unroll, grid_max, flags=0):
+        call("hpb_set_tuning", self._h, kind, 0 if np.dtype(dtype) == np.float64 else 1, block, unroll, grid_max,
+             flags)
 
     def get_tuning(self, kind, dtype):
-        b, u, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        b, u, g, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         call("hpb_get_tuning", self._h, kind, 0 if np.dtype(dtype) == np.float64 else 1,
-             ctypes.byref(b), ctypes.byref(u), ctypes.byref(g))
-        return b.value, u.value, g.value
+             ctypes.byref(b), ctypes.byref(u), ctypes.byref(g), ctypes.byref(f))
+        return b.value, u.value, g.value, f.value
 
     def reduce_config(self, dtype, n, aligned=True):
         cfg = (ctypes.c_int * 4)()

@@ -35,11 +35,13 @@ const char* hpb_version(void) { return "hpb 0.1 gfx950"; }
 
 static void default_tuning(hpb_handle_s* h)
 {
-    // Reductions: 256 threads, 4 x 16-byte loads per input per tile, grid capped
-    // at 4 blocks per CU.  Elementwise: 256 threads, 2 vectors per thread.
+    // Reductions: 256 threads, 4 x 16-byte loads per input per tile, one block
+    // per CU (measured fastest at N = 2^27 on MI355X: more resident blocks lower
+    // the achieved bandwidth).  Elementwise: 256 threads, 2 vectors per thread.
+    const int cu = h->num_cu > 0 ? h->num_cu : 256;
     for (int d = 0; d < 2; ++d) {
-        h->tune_red[d] = {256, 4, 4 * (h->num_cu > 0 ? h->num_cu : 256)};
-        h->tune_map[d] = {256, 2, 8 * (h->num_cu > 0 ? h->num_cu : 256)};
+        h->tune_red[d] = {256, 4, cu, 0};
+        h->tune_map[d] = {256, 2, 8 * cu, 0};
     }
 }
 
@@ -162,7 +164,7 @@ int hpb_memcpy_d2d(hpb_handle_t h, void* dst, const void* src, size_t bytes)
     return HPB_OK;
 }
 
-int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max)
+int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max, int flags)
 {
     HPB_REQUIRE(h, HPB_ERR_INVALID, "hpb_set_tuning: null handle");
     HPB_REQUIRE(kind == 0 || kind == 1, HPB_ERR_INVALID, "hpb_set_tuning: kind must be 0 or 1");
@@ -171,13 +173,14 @@ int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, i
     HPB_REQUIRE(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, HPB_ERR_UNSUPPORTED,
                 "hpb_set_tuning: unroll must be 1, 2, 4 or 8");
     HPB_REQUIRE(grid_max >= 1 && grid_max <= HPB_MAX_GRID, HPB_ERR_INVALID, "hpb_set_tuning: grid_max out of range");
-    hpb_tuning_t t = {block, unroll, grid_max};
+    HPB_REQUIRE((flags & ~HPB_TUNE_NONTEMPORAL) == 0, HPB_ERR_UNSUPPORTED, "hpb_set_tuning: unknown flags");
+    hpb_tuning_t t = {block, unroll, grid_max, flags};
     if (kind == 0) h->tune_red[dtype] = t;
     else h->tune_map[dtype] = t;
     return HPB_OK;
 }
 
-int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max)
+int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max, int* flags)
 {
     HPB_REQUIRE(h && block && unroll && grid_max, HPB_ERR_INVALID, "hpb_get_tuning: null argument");
     HPB_REQUIRE((kind == 0 || kind == 1) && (dtype == 0 || dtype == 1), HPB_ERR_INVALID, "hpb_get_tuning: bad kind/dtype");
@@ -185,6 +188,7 @@ int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll,
     *block = t.block;
     *unroll = t.unroll;
     *grid_max = t.grid_max;
+    if (flags) *flags = t.flags;
     return HPB_OK;
 }
 

@@ -11,6 +11,7 @@ struct hpb_tuning_t {
     int block;      // threads per block (multiple of 64)
     int unroll;     // 16-byte vectors per thread per tile
     int grid_max;   // cap on the number of blocks (grid-stride beyond it)
+    int flags;      // HPB_TUNE_* bits
 };
 
 struct hpb_handle_s {

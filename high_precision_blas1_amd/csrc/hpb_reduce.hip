@@ -116,7 +116,32 @@ __device__ __forceinline__ bool take_ticket(unsigned* ticket, unsigned G)
     return last;
 }
 
-template <typename T, int OP, int BLOCK, int UNROLL, bool VEC>
+template <typename V>
+__device__ __forceinline__ V load_nt(const V* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ double2 load_nt<double2>(const double2* p)
+{
+    double2 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+}
+template <>
+__device__ __forceinline__ float4 load_nt<float4>(const float4* p)
+{
+    float4 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    v.z = __builtin_nontemporal_load(&p->z);
+    v.w = __builtin_nontemporal_load(&p->w);
+    return v;
+}
+
+// NT: stream the full tiles with nontemporal loads (no reuse is expected).
+template <typename T, int OP, int BLOCK, int UNROLL, bool VEC, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict__ x, const T* __restrict__ y,
                                                   T* __restrict__ part, unsigned* __restrict__ ticket,
                                                   T* __restrict__ out)
@@ -140,8 +165,13 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict_
         V xa[UNROLL], ya[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            xa[u] = xv[base + (size_t)u * BLOCK];
-            if constexpr (tr::binary) ya[u] = yv[base + (size_t)u * BLOCK];
+            if constexpr (NT) {
+                xa[u] = load_nt<V>(xv + base + (size_t)u * BLOCK);
+                if constexpr (tr::binary) ya[u] = load_nt<V>(yv + base + (size_t)u * BLOCK);
+            } else {
+                xa[u] = xv[base + (size_t)u * BLOCK];
+                if constexpr (tr::binary) ya[u] = yv[base + (size_t)u * BLOCK];
+            }
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -200,7 +230,7 @@ template <> struct dtype_id<double> { static constexpr int v = 0; };
 template <> struct dtype_id<float> { static constexpr int v = 1; };
 
 struct red_shape {
-    int block, unroll, grid, vecw;
+    int block, unroll, grid, vecw, flags;
 };
 
 template <typename T>
@@ -210,6 +240,7 @@ static red_shape reduce_shape(const hpb_handle_s* h, size_t n, bool aligned)
     red_shape r;
     r.block = t.block;
     r.unroll = t.unroll;
+    r.flags = t.flags;
     r.vecw = aligned ? dev::vec16<T>::W : 1;
     const size_t nv = n / (size_t)r.vecw;
     const size_t tile = (size_t)t.block * (size_t)t.unroll;
@@ -223,7 +254,10 @@ template <typename T, int OP, int BLOCK, int UNROLL>
 static void launch_bu(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
                       unsigned* ticket, T* out)
 {
-    if (sh.vecw > 1)
+    if (sh.vecw > 1 && (sh.flags & HPB_TUNE_NONTEMPORAL) && UNROLL >= 2)
+        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true, true>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y,
+                           part, ticket, out);
+    else if (sh.vecw > 1)
         hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y, part,
                            ticket, out);
     else

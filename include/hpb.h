@@ -72,8 +72,9 @@ int hpb_memcpy_d2d(hpb_handle_t h, void* dev_dst, const void* dev_src, size_t by
 /* kind: 0 reductions, 1 elementwise; dtype: 0 f64, 1 f32.  The autotuner
  * (high_precision_blas1_amd/autotune.py) writes these; BLOCK_SIZE=1024 is a fixed
  * template parameter in the reference (gpu_reduction_ogita.h:36). */
-int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max);
-int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max);
+#define HPB_TUNE_NONTEMPORAL 1 /* reductions: nontemporal loads for the streamed tiles */
+int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max, int flags);
+int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max, int* flags);
 /* Launch shape a reduction of n elements would use: cfg = {block, unroll, grid,
  * vector width (1 if the pointers are not 16-byte aligned)}. */
 int hpb_reduce_config(hpb_handle_t h, int dtype, size_t n, int aligned16, int* cfg4);
@@ -205,6 +206,63 @@ int hpb_random_uniform_f32(hpb_handle_t h, size_t n, uint64_t seed, uint32_t str
  * FP32/FP64 boundary of the refined solver). */
 int hpb_convert_f32_to_f64(hpb_handle_t h, size_t n, const float* x, double* y);
 int hpb_convert_f64_to_f32(hpb_handle_t h, size_t n, const double* x, float* y);
+
+/* ---- CSR sparse matrices (csr::matrix, source/common/csr/matrix.h:91-158:
+ *      zero-based int32 row_ptr[rows+1], col_ind[nnz], values[nnz]) ---------- */
+/* y := alpha*A*x + beta*y; beta == 0 does not read y.  Replaces
+ * csr::gpu_matrix::axpy (gpu_matrix.h:322-375, cusparseSpMV) as called by
+ * csr::linear_operator::apply (csr/linear_operator.h:39-42, alpha 1, beta 0).
+ * Plain rows accumulate s = fma(a_ij, x_j, s). */
+int hpb_csr_spmv_f64(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                     const double* val, double alpha, const double* x, double beta, double* y);
+int hpb_csr_spmv_f32(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                     const float* val, float alpha, const float* x, float beta, float* y);
+/* Compensated-row SpMV (new, no reference counterpart): every row product is an
+ * Ogita-Rump-Oishi Dot2, so y_i is accurate to u|row_i.x| + gamma^2 |row_i|.|x|. */
+int hpb_csr_spmvc_f64(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                      const double* val, double alpha, const double* x, double beta, double* y);
+int hpb_csr_spmvc_f32(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                      const float* val, float alpha, const float* x, float beta, float* y);
+/* r := b - A*x in one pass (the solvers' calc_residual_, gmres.h:122-130, fused);
+ * the c variant enters b_i as the first term of a compensated row sum. */
+int hpb_csr_residual_f64(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                         const double* val, const double* b, const double* x, double* r);
+int hpb_csr_residualc_f64(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                          const double* val, const double* b, const double* x, double* r);
+/* Same as hpb_csr_spmv(c)_f64 with an explicit number of lanes per row
+ * (1, 2, ..., 64; 0 = chosen from the mean row length) -- for tuning and tests. */
+int hpb_csr_spmv_lpr_f64(hpb_handle_t h, int lanes_per_row, int compensated, int rows, int cols, int64_t nnz,
+                         const int* row_ptr, const int* col_ind, const double* val, double alpha, const double* x,
+                         double beta, double* y);
+
+/* ---- structured test matrices (the benchmark problems; the reference reads
+ *      MatrixMarket files instead, csr/matrix_market_reader.h:246) ---------- */
+/* rows and non-zeros of an nx x ny x nz 7-point (nz == 1: 5-point) stencil with
+ * Dirichlet boundary neighbours dropped. */
+int hpb_stencil_nnz(int nx, int ny, int nz, int64_t* out_rows, int64_t* out_nnz);
+/* Constant-coefficient stencil, rows idx = i + nx*(j + ny*k), columns ascending;
+ * coef7 (host) = { z-, y-, x-, diagonal, x+, y+, z+ }.  Poisson: {-1,-1,-1,6,-1,-1,-1}. */
+int hpb_csr_stencil7_f64(hpb_handle_t h, int nx, int ny, int nz, const double* coef7, int* row_ptr, int* col_ind,
+                         double* val);
+/* kappa_c = exp(sigma g_c), g_c ~ N(0,1) by Box-Muller from Philox (key seed, counter c). */
+int hpb_lognormal_field_f64(hpb_handle_t h, int64_t ncell, double sigma, uint64_t seed, double* kappa);
+/* 5-point diffusion on m x m cells: face coefficient = harmonic mean 2ab/(a+b)
+ * of the two cells (the cell's own kappa on boundary faces); off-diagonal -face,
+ * diagonal = sum of the four faces ((s + w) + e) + n. */
+int hpb_csr_diffusion2d_f64(hpb_handle_t h, int m, const double* kappa, int* row_ptr, int* col_ind, double* val);
+
+/* ---- ill-conditioned pair generator -------------------------------------- */
+/* Device-side, seeded GenDot pair (Ogita-Rump-Oishi 2005 Alg. 6.1) replacing
+ * generate_vector_pair::generate (generate_vector_pair.hpp:119-182, host-driven,
+ * std::random_device seeded, O(N_fix N) Dot2 calls).  Head n - n_tail entries:
+ * x_i, y_i = m 2^e with e uniform in [0, log2(cond)/2]; tail: exponents descending
+ * to 0, y chosen to cancel the running dot.  The exact dot is tracked in 256-bit
+ * fixed point (LSB 2^-106): out_exact4 receives its two's complement limbs
+ * (least significant first), out3 = { round(x.y), round(sum |x_i y_i|), kappa }.
+ * Bit-identical to the host restatement for the same arguments.  Requires
+ * 2 <= n, 1 <= n_tail < n, cond >= 1, log2(cond) + log2(n) <= 143. Blocking. */
+int hpb_gendot_f64(hpb_handle_t h, size_t n, double log2cond, uint64_t seed, size_t n_tail, double* x, double* y,
+                   uint64_t* out_exact4, double* out3);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,8 @@ def lib():
     vp = ctypes.c_void_p
     _sig(L, "orc_map", None, [ctypes.c_int, _sz, vp, vp, vp, vp, vp, vp])
     _sig(L, "orc_map_f32", None, [ctypes.c_int, _sz, vp, vp, vp, vp, vp, vp])
+    _sig(L, "orc_csr_spmv", None, [ctypes.c_int, vp, vp, vp, vp, d, d, vp, vp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int])
     _LIB = L
     return L
 
@@ -241,3 +243,37 @@ def map_op(op, n, scalars, inputs, dtype=np.float64):
     f = lib().orc_map if dtype == np.float64 else lib().orc_map_f32
     f(MAP_OPS[op], n, s.ctypes.data, *[None if a is None else a.ctypes.data for a in ins], out.ctypes.data)
     return out
+
+
+def csr_spmv(A, x, alpha=1.0, beta=0.0, y=None, lpr=1, comp=False, b=None):
+    """y = alpha A x + beta y (or b - A x when b is given) in the library's row
+    order; A is a scipy.sparse CSR matrix with int32 indices."""
+    rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    col = np.ascontiguousarray(A.indices, dtype=np.int32)
+    val = np.ascontiguousarray(A.data, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rows = A.shape[0]
+    out = np.zeros(rows) if y is None else np.array(y, dtype=np.float64)
+    bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+    lib().orc_csr_spmv(rows, rp.ctypes.data, col.ctypes.data, val.ctypes.data, x.ctypes.data, alpha, beta,
+                       None if bb is None else bb.ctypes.data, out.ctypes.data, int(lpr), int(comp),
+                       0 if b is None else 1)
+    return out
+
+
+def stencil7(nx, ny, nz, coef):
+    """CSR of a constant 7-point stencil built independently with scipy Kronecker
+    products; coef = (z-, y-, x-, diag, x+, y+, z+), lexicographic i fastest."""
+    import scipy.sparse as sp
+    zm, ym, xm, d, xp, yp, zp = coef
+    def one(n, lo, up):
+        return sp.diags([np.full(n - 1, lo), np.full(n - 1, up)], [-1, 1], shape=(n, n), format="csr")
+    Ix, Iy, Iz = (sp.identity(m, format="csr") for m in (nx, ny, nz))
+    A = sp.kron(Iz, sp.kron(Iy, one(nx, xm, xp))) + sp.kron(Iz, sp.kron(one(ny, ym, yp), Ix))
+    if nz > 1:
+        A = A + sp.kron(one(nz, zm, zp), sp.kron(Iy, Ix))
+    A = (A + d * sp.identity(nx * ny * nz)).tocsr()
+    A.sort_indices()
+    A.indptr = A.indptr.astype(np.int32)
+    A.indices = A.indices.astype(np.int32)
+    return A

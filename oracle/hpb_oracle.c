@@ -1038,3 +1038,55 @@ ORC_API void orc_map##SUF(int op, size_t n, const T *s, const T *i0, const T *i1
 }
 MAP_DEFINE(double, , two_sum, two_prod, fma, fabs)
 MAP_DEFINE(float, _f32, two_sumf, two_prodf, fmaf, fabsf)
+
+/* ------------------------------------------------------------------------ */
+/* CSR products                                                              */
+/* ------------------------------------------------------------------------ */
+/* y = alpha A x + beta y (mode 0) or r = b - A x (mode 1) for zero-based CSR
+ * (source/common/csr/matrix.h:91-158), evaluated in the library's row order:
+ * lane l of lpr lanes takes entries beg+l, beg+l+lpr, ...; plain lanes
+ * accumulate s = fma(a, x, s), compensated lanes run the Dot2 step; lanes are
+ * merged by an xor butterfly (distance lpr/2 .. 1, lane 0's value kept); mode 1
+ * starts lane 0 at b_i and uses -a.  The result: plain s, compensated fl(s + c);
+ * mode 0 then y = v (beta 0, alpha 1), alpha v (beta 0) or fma(alpha, v, beta y). */
+ORC_API void orc_csr_spmv(int rows, const int *rp, const int *col, const double *val, const double *x,
+                          double alpha, double beta, const double *b, double *y, int lpr, int comp, int mode)
+{
+    double ls[64], lc[64], ns[64], nc[64];
+    for (int r = 0; r < rows; ++r) {
+        for (int l = 0; l < lpr; ++l) {
+            double s = (mode == 1 && l == 0) ? b[r] : 0.0, c = 0.0;
+            for (int k = rp[r] + l; k < rp[r + 1]; k += lpr) {
+                double a = mode == 1 ? -val[k] : val[k];
+                if (comp) {
+                    double pi, t, p = two_prod(a, x[col[k]], &pi);
+                    s = two_sum(s, p, &t);
+                    c = c + (pi + t);
+                } else {
+                    s = fma(a, x[col[k]], s);
+                }
+            }
+            ls[l] = s;
+            lc[l] = c;
+        }
+        for (int m = lpr / 2; m >= 1; m >>= 1) {
+            for (int l = 0; l < lpr; ++l) {
+                ns[l] = ls[l];
+                nc[l] = lc[l];
+                if (comp) {
+                    double t;
+                    ns[l] = two_sum(ls[l], ls[l ^ m], &t);
+                    nc[l] = (lc[l] + t) + lc[l ^ m];
+                } else {
+                    ns[l] = ls[l] + ls[l ^ m];
+                }
+            }
+            memcpy(ls, ns, sizeof(double) * (size_t)lpr);
+            memcpy(lc, nc, sizeof(double) * (size_t)lpr);
+        }
+        double v = comp ? ls[0] + lc[0] : ls[0];
+        if (mode == 1) y[r] = v;
+        else if (beta == 0.0) y[r] = (alpha == 1.0) ? v : alpha * v;
+        else y[r] = fma(alpha, v, beta * y[r]);
+    }
+}

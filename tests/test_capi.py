@@ -48,7 +48,7 @@ def test_invalid_arguments_fail_loudly_without_device_work():
     assert "null" in L.hpb_last_error().decode()
     assert L.hpb_create(None, None) == -1
     with pytest.raises(H.HPBError):
-        H.call("hpb_set_tuning", None, 0, 0, 256, 4, 16)
+        H.call("hpb_set_tuning", None, 0, 0, 256, 4, 16, 0)
 
 
 def test_header_is_c_compatible(tmp_path):

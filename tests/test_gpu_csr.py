@@ -1,0 +1,182 @@
+"""GPU CSR: structured-matrix builders bit-equal to independent scipy
+construction, SpMV / residual bit-exact against the oracle's row order for every
+lanes-per-row split, and size-independent checks at 256^3 (closed-form A*1,
+exact power-of-two linearity)."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import high_precision_blas1_amd as H
+import oracle as O
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+POISSON = (-1.0, -1.0, -1.0, 6.0, -1.0, -1.0, -1.0)
+# first-order upwind convection-diffusion, velocity (1, 0.5, 0.25), cell Peclet 100,
+# scaled by h^2/nu: lower neighbour -1 - Pe_d, upper -1, diagonal 6 + sum Pe_d
+CONVDIFF = (-26.0, -51.0, -101.0, 181.0, -1.0, -1.0, -1.0)
+ANISO = (-1e4, -1.0, -1.0, 4.0 + 2e4, -1.0, -1.0, -1e4)
+
+
+class DevCSR:
+    def __init__(self, handle, rows, nnz):
+        self.rows, self.nnz = rows, nnz
+        self.rp = H.DeviceArray(rows + 1, np.int32, handle)
+        self.col = H.DeviceArray(nnz, np.int32, handle)
+        self.val = H.DeviceArray(nnz, np.float64, handle)
+
+    def free(self):
+        for a in (self.rp, self.col, self.val):
+            a.free()
+
+    def host(self):
+        return sp.csr_matrix((self.val.numpy(), self.col.numpy(), self.rp.numpy()), shape=(self.rows, self.rows))
+
+
+def build_stencil(handle, nx, ny, nz, coef):
+    rows, nnz = ctypes.c_int64(), ctypes.c_int64()
+    H.call("hpb_stencil_nnz", nx, ny, nz, ctypes.byref(rows), ctypes.byref(nnz))
+    A = DevCSR(handle, rows.value, nnz.value)
+    c = np.array(coef, dtype=np.float64)
+    H.call("hpb_csr_stencil7_f64", handle.ptr, nx, ny, nz, c.ctypes.data_as(ctypes.c_void_p), A.rp.ptr, A.col.ptr,
+           A.val.ptr)
+    return A
+
+
+def upload_csr(handle, M):
+    A = DevCSR.__new__(DevCSR)
+    A.rows, A.nnz = M.shape[0], M.nnz
+    A.rp = H.DeviceArray.from_numpy(M.indptr.astype(np.int32), handle)
+    A.col = H.DeviceArray.from_numpy(M.indices.astype(np.int32), handle)
+    A.val = H.DeviceArray.from_numpy(M.data.astype(np.float64), handle)
+    return A
+
+
+def spmv(handle, A, x, alpha=1.0, beta=0.0, y=None, lpr=0, comp=False):
+    if y is None:
+        y = H.DeviceArray(A.rows, np.float64, handle)
+    H.call("hpb_csr_spmv_lpr_f64", handle.ptr, lpr, int(comp), A.rows, A.rows, A.nnz, A.rp.ptr, A.col.ptr,
+           A.val.ptr, ctypes.c_double(alpha), x.ptr, ctypes.c_double(beta), y.ptr)
+    return y
+
+
+@pytest.mark.parametrize("dims", [(7, 5, 3), (16, 16, 16), (33, 17, 1), (1, 1, 9), (64, 1, 1)])
+@pytest.mark.parametrize("coef", [POISSON, CONVDIFF, ANISO])
+def test_stencil_builder_matches_scipy(handle, dims, coef):
+    say("stencil", dims, coef)
+    A = build_stencil(handle, *dims, coef)
+    ref = O.stencil7(*dims, coef)
+    got = A.host()
+    assert np.array_equal(got.indptr, ref.indptr)
+    assert np.array_equal(got.indices, ref.indices)
+    assert got.data.tobytes() == ref.data.astype(np.float64).tobytes()
+    A.free()
+
+
+def test_diffusion2d_builder(handle):
+    m = 40
+    kap = H.DeviceArray(m * m, np.float64, handle)
+    H.call("hpb_lognormal_field_f64", handle.ptr, m * m, ctypes.c_double(1.0), 3, kap.ptr)
+    k = kap.numpy()
+    assert np.all(k > 0)
+    rows, nnz = ctypes.c_int64(), ctypes.c_int64()
+    H.call("hpb_stencil_nnz", m, m, 1, ctypes.byref(rows), ctypes.byref(nnz))
+    A = DevCSR(handle, rows.value, nnz.value)
+    H.call("hpb_csr_diffusion2d_f64", handle.ptr, m, kap.ptr, A.rp.ptr, A.col.ptr, A.val.ptr)
+    got = A.host()
+    # independent host assembly from the same field
+    K = k.reshape(m, m)  # [j, i]
+    def face(a, b):
+        return 2.0 * a * b / (a + b)
+    rows_l, cols_l, vals_l = [], [], []
+    for j in range(m):
+        for i in range(m):
+            r = i + m * j
+            kc = K[j, i]
+            fs = face(kc, K[j - 1, i]) if j > 0 else kc
+            fw = face(kc, K[j, i - 1]) if i > 0 else kc
+            fe = face(kc, K[j, i + 1]) if i < m - 1 else kc
+            fn = face(kc, K[j + 1, i]) if j < m - 1 else kc
+            ent = []
+            if j > 0: ent.append((r - m, -fs))
+            if i > 0: ent.append((r - 1, -fw))
+            ent.append((r, ((fs + fw) + fe) + fn))
+            if i < m - 1: ent.append((r + 1, -fe))
+            if j < m - 1: ent.append((r + m, -fn))
+            for c_, v_ in ent:
+                rows_l.append(r); cols_l.append(c_); vals_l.append(v_)
+    ref = sp.csr_matrix((vals_l, (rows_l, cols_l)), shape=(m * m, m * m))
+    ref.sort_indices()
+    assert np.array_equal(got.indices, ref.indices) and np.array_equal(got.indptr, ref.indptr)
+    assert got.data.tobytes() == np.asarray(ref.data, dtype=np.float64).tobytes()
+    assert abs(got - got.T).max() == 0.0
+    big = H.DeviceArray(1 << 20, np.float64, handle)
+    H.call("hpb_lognormal_field_f64", handle.ptr, 1 << 20, ctypes.c_double(1.0), 3, big.ptr)
+    g = np.log(big.numpy())
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1.0) < 0.01
+
+
+@pytest.mark.parametrize("lpr", [0, 1, 2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("comp", [False, True])
+def test_spmv_bitexact(handle, lpr, comp):
+    r = np.random.default_rng(lpr + 10 * comp)
+    M = sp.random(3000, 3000, density=0.004, format="csr", random_state=lpr)
+    M.data = r.standard_normal(M.nnz) * np.exp2(r.integers(-20, 20, M.nnz))
+    M = M.tocsr()
+    M.sort_indices()
+    cases = [M, O.stencil7(12, 10, 9, CONVDIFF)]
+    for Mh in cases:
+        say("spmv", lpr, comp, Mh.shape, Mh.nnz)
+        A = upload_csr(handle, Mh)
+        x = r.standard_normal(Mh.shape[0])
+        dx = H.DeviceArray.from_numpy(x, handle)
+        eff = lpr if lpr else {True: 4, False: 4}[True]
+        if lpr == 0:
+            mean = Mh.nnz / Mh.shape[0]
+            eff = 1 if mean <= 1.5 else 2 if mean <= 3 else 4 if mean <= 6 else 8 if mean <= 12 else 16
+        y = spmv(handle, A, dx, lpr=lpr, comp=comp).numpy()
+        assert y.tobytes() == O.csr_spmv(Mh, x, lpr=eff, comp=comp).tobytes()
+        y0 = r.standard_normal(Mh.shape[0])
+        dy = H.DeviceArray.from_numpy(y0, handle)
+        spmv(handle, A, dx, alpha=-0.5, beta=2.0, y=dy, lpr=lpr, comp=comp)
+        assert dy.numpy().tobytes() == O.csr_spmv(Mh, x, -0.5, 2.0, y0, lpr=eff, comp=comp).tobytes()
+        A.free()
+
+
+def test_residual_bitexact(handle):
+    Mh = O.stencil7(20, 20, 20, CONVDIFF)
+    A = upload_csr(handle, Mh)
+    r = np.random.default_rng(2)
+    x, b = r.standard_normal(Mh.shape[0]), r.standard_normal(Mh.shape[0])
+    dx, db = H.DeviceArray.from_numpy(x, handle), H.DeviceArray.from_numpy(b, handle)
+    out = H.DeviceArray(Mh.shape[0], np.float64, handle)
+    for name, comp in (("hpb_csr_residual_f64", False), ("hpb_csr_residualc_f64", True)):
+        H.call(name, handle.ptr, A.rows, A.rows, A.nnz, A.rp.ptr, A.col.ptr, A.val.ptr, db.ptr, dx.ptr, out.ptr)
+        assert out.numpy().tobytes() == O.csr_spmv(Mh, x, lpr=8, comp=comp, b=b).tobytes()
+
+
+def test_poisson_256_cubed_properties(handle):
+    nx = 256
+    say("poisson 256^3")
+    A = build_stencil(handle, nx, nx, nx, POISSON)
+    n = A.rows
+    assert n == nx ** 3 and A.nnz == n + 6 * (nx - 1) * nx * nx
+    one = H.DeviceArray(n, np.float64, handle)
+    H.call("hpb_fill_f64", handle.ptr, n, ctypes.c_double(1.0), one.ptr)
+    for comp in (False, True):
+        y = spmv(handle, A, one, comp=comp).numpy()
+        i = np.arange(n) % nx
+        j = (np.arange(n) // nx) % nx
+        k = np.arange(n) // (nx * nx)
+        missing = ((i == 0).astype(np.int8) + (i == nx - 1) + (j == 0) + (j == nx - 1) + (k == 0) + (k == nx - 1))
+        assert np.array_equal(y, missing.astype(np.float64))
+    x = H.DeviceArray(n, np.float64, handle)
+    H.call("hpb_random_uniform_f64", handle.ptr, n, 1, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), x.ptr)
+    y1 = spmv(handle, A, x, comp=True).numpy()
+    H.call("hpb_scal_f64", handle.ptr, n, ctypes.c_double(2.0), x.ptr)
+    y2 = spmv(handle, A, x, comp=True).numpy()
+    assert np.array_equal(y2, 2.0 * y1)
+    A.free()

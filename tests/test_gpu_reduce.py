@@ -72,10 +72,11 @@ def test_bitexact_all_launch_shapes(handle, dtype):
         for block in (256, 512):
             for unroll in (1, 2, 4, 8):
                 for gmax in (1, 3, 64, 1024, 4096):
-                    handle.set_tuning(0, dtype, block, unroll, gmax)
-                    say("shape", block, unroll, gmax, np.dtype(dtype).name)
-                    _check_op(handle, "dot2", x, y, 0)
-                    _check_op(handle, "sum2", x, y, 0)
+                    for flags in (0, 1):
+                        handle.set_tuning(0, dtype, block, unroll, gmax, flags)
+                        say("shape", block, unroll, gmax, flags, np.dtype(dtype).name)
+                        _check_op(handle, "dot2", x, y, 0)
+                        _check_op(handle, "sum2", x, y, 0)
     finally:
         handle.set_tuning(0, dtype, *saved)
 
