@@ -1,0 +1,130 @@
+// hip_matrix.h -- CSR matrix on the device with SpMV through the gfx950 library.
+//
+// Interface of csr::matrix / csr::gpu_matrix (source/common/csr/matrix.h:86-140,
+// source/common/csr/gpu_matrix.h:119-131, 322-375): init(nnz, size_col,
+// size_row), set(data, col_ind, row_ptr) from host arrays, init_set, get_dim,
+// axpy(alpha, x, beta, y, transpose) computing y := alpha*A*x + beta*y.
+// Zero-based int32 ordinals like the reference's default.  Unlike the reference
+// there is no per-call allocation; rows can be evaluated as compensated dot
+// products (use_compensated_rows), and a fused residual r := b - A x is offered.
+// Transposed products are not supported (no solver in the path uses them) and
+// throw std::logic_error.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <vector>
+
+#include "../hip_vector_operations.h"
+
+namespace csr {
+
+template <typename T>
+class hip_matrix {
+public:
+    struct dimensions_t {
+        int columns = 0;
+        int rows = 0;
+    };
+    using scalar_type = T;
+    using vector_type = T*;
+
+    explicit hip_matrix(hpb_handle_t h) : h_(h) {}
+    hip_matrix(const hip_matrix&) = delete;
+    hip_matrix& operator=(const hip_matrix&) = delete;
+    ~hip_matrix() { release_(); }
+
+    void set_dim(int matrix_columns, int matrix_rows)
+    {
+        dim_.columns = matrix_columns;
+        dim_.rows = matrix_rows;
+    }
+    dimensions_t get_dim() const { return dim_; }
+
+    // sizes as in the reference: nnz, size_col = length of col_ind (= nnz), size_row = rows + 1
+    void init(int64_t nnz, int64_t size_col, int64_t size_row)
+    {
+        if (initialized_) return;
+        nnz_ = nnz;
+        size_col_ = size_col;
+        size_row_ = size_row;
+        hpb::check(hpb_malloc((void**)&data_, sizeof(T) * (size_t)nnz_), "csr::hip_matrix::init");
+        hpb::check(hpb_malloc((void**)&col_ind_, sizeof(int) * (size_t)size_col_), "csr::hip_matrix::init");
+        hpb::check(hpb_malloc((void**)&row_ptr_, sizeof(int) * (size_t)size_row_), "csr::hip_matrix::init");
+        if (dim_.rows == 0) set_dim((int)(size_row_ - 1), (int)(size_row_ - 1));
+        initialized_ = true;
+    }
+    void set(const T* data, const int* col_ind, const int* row_ptr)
+    {
+        if (!initialized_) throw std::runtime_error("csr::matrix: trying to set an uninitialized matrix.");
+        hpb::check(hpb_memcpy_h2d(h_, data_, data, sizeof(T) * (size_t)nnz_), "csr::hip_matrix::set");
+        hpb::check(hpb_memcpy_h2d(h_, col_ind_, col_ind, sizeof(int) * (size_t)size_col_), "csr::hip_matrix::set");
+        hpb::check(hpb_memcpy_h2d(h_, row_ptr_, row_ptr, sizeof(int) * (size_t)size_row_), "csr::hip_matrix::set");
+    }
+    void init_set(int64_t nnz, int64_t size_col, int64_t size_row, const T* data, const int* col_ind,
+                  const int* row_ptr, int matrix_columns, int matrix_rows)
+    {
+        set_dim(matrix_columns, matrix_rows);
+        init(nnz, size_col, size_row);
+        set(data, col_ind, row_ptr);
+    }
+    // adopt device arrays (e.g. from the library's stencil builders); own = false
+    // leaves freeing them to the caller (views sharing the index arrays)
+    void adopt_device(int rows, int cols, int64_t nnz, int* row_ptr, int* col_ind, T* data, bool own = true)
+    {
+        release_();
+        set_dim(cols, rows);
+        nnz_ = nnz;
+        size_col_ = nnz;
+        size_row_ = (int64_t)rows + 1;
+        row_ptr_ = row_ptr;
+        col_ind_ = col_ind;
+        data_ = data;
+        own_ = own;
+        initialized_ = true;
+    }
+
+    void use_compensated_rows(bool v) { comp_ = v; }
+    bool compensated_rows() const { return comp_; }
+
+    // y := alpha*A*x + beta*y
+    void axpy(const T& alpha, const vector_type& x, const T& beta, vector_type& y, bool transpose = false) const
+    {
+        if (transpose) throw std::logic_error("csr::hip_matrix::axpy: transposed product not supported");
+        auto fn = comp_ ? hpb::abi<T>::csr_spmvc : hpb::abi<T>::csr_spmv;
+        hpb::check(fn(h_, dim_.rows, dim_.columns, nnz_, row_ptr_, col_ind_, data_, alpha, x, beta, y),
+                   "csr::hip_matrix::axpy");
+    }
+
+    int rows() const { return dim_.rows; }
+    int64_t nnz() const { return nnz_; }
+    const int* row_ptr() const { return row_ptr_; }
+    const int* col_ind() const { return col_ind_; }
+    const T* data() const { return data_; }
+    hpb_handle_t handle() const { return h_; }
+
+private:
+    void release_()
+    {
+        if (own_) {
+            if (data_) hpb_free(data_);
+            if (col_ind_) hpb_free(col_ind_);
+            if (row_ptr_) hpb_free(row_ptr_);
+        }
+        own_ = true;
+        data_ = nullptr;
+        col_ind_ = nullptr;
+        row_ptr_ = nullptr;
+        initialized_ = false;
+    }
+    hpb_handle_t h_;
+    dimensions_t dim_;
+    int64_t nnz_ = 0, size_col_ = 0, size_row_ = 0;
+    T* data_ = nullptr;
+    int* col_ind_ = nullptr;
+    int* row_ptr_ = nullptr;
+    bool initialized_ = false;
+    bool comp_ = false;
+    bool own_ = true;
+};
+
+}  // namespace csr

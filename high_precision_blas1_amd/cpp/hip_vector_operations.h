@@ -37,23 +37,29 @@ template <typename T> struct abi;
 
 template <> struct abi<double> {
 #define HPB_F(name) static constexpr auto name = &hpb_##name##_f64;
-    HPB_F(dot2) HPB_F(dot) HPB_F(dot2_async) HPB_F(dot_async) HPB_F(nrm2c) HPB_F(nrm2) HPB_F(nrm2c_async)
-    HPB_F(nrm2_async) HPB_F(asum2) HPB_F(asum) HPB_F(sum2) HPB_F(sum) HPB_F(fill) HPB_F(add_mul_scalar) HPB_F(scal)
+#define HPB_FA(name) static constexpr auto name##_async = &hpb_##name##_f64_async;
+    HPB_F(dot2) HPB_F(dot) HPB_FA(dot2) HPB_FA(dot) HPB_F(nrm2c) HPB_F(nrm2) HPB_FA(nrm2c)
+    HPB_FA(nrm2) HPB_F(asum2) HPB_F(asum) HPB_F(sum2) HPB_F(sum) HPB_F(fill) HPB_F(add_mul_scalar) HPB_F(scal)
     HPB_F(copy) HPB_F(swap) HPB_F(abs) HPB_F(assign_mul) HPB_F(axpy) HPB_F(axpby) HPB_F(axpbypcz)
     HPB_F(axpbypcwpdz) HPB_F(assign_mul4) HPB_F(mul_pointwise) HPB_F(mul_pointwise_inplace) HPB_F(div_pointwise)
     HPB_F(div_pointwise_inplace) HPB_F(axpbyc) HPB_F(valid) HPB_F(set_value) HPB_F(get_value)
-    HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc)
+    HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc) HPB_F(multidot) HPB_F(multiaxpy) HPB_FA(multidot)
+    HPB_F(multiaxpy_dev)
 #undef HPB_F
+#undef HPB_FA
 };
 template <> struct abi<float> {
 #define HPB_F(name) static constexpr auto name = &hpb_##name##_f32;
-    HPB_F(dot2) HPB_F(dot) HPB_F(dot2_async) HPB_F(dot_async) HPB_F(nrm2c) HPB_F(nrm2) HPB_F(nrm2c_async)
-    HPB_F(nrm2_async) HPB_F(asum2) HPB_F(asum) HPB_F(sum2) HPB_F(sum) HPB_F(fill) HPB_F(add_mul_scalar) HPB_F(scal)
+#define HPB_FA(name) static constexpr auto name##_async = &hpb_##name##_f32_async;
+    HPB_F(dot2) HPB_F(dot) HPB_FA(dot2) HPB_FA(dot) HPB_F(nrm2c) HPB_F(nrm2) HPB_FA(nrm2c)
+    HPB_FA(nrm2) HPB_F(asum2) HPB_F(asum) HPB_F(sum2) HPB_F(sum) HPB_F(fill) HPB_F(add_mul_scalar) HPB_F(scal)
     HPB_F(copy) HPB_F(swap) HPB_F(abs) HPB_F(assign_mul) HPB_F(axpy) HPB_F(axpby) HPB_F(axpbypcz)
     HPB_F(axpbypcwpdz) HPB_F(assign_mul4) HPB_F(mul_pointwise) HPB_F(mul_pointwise_inplace) HPB_F(div_pointwise)
     HPB_F(div_pointwise_inplace) HPB_F(axpbyc) HPB_F(valid) HPB_F(set_value) HPB_F(get_value)
-    HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc)
+    HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc) HPB_F(multidot) HPB_F(multiaxpy) HPB_FA(multidot)
+    HPB_F(multiaxpy_dev)
 #undef HPB_F
+#undef HPB_FA
 };
 
 template <typename T>
@@ -263,6 +269,24 @@ struct hip_vector_operations {
                                 const vector_type& y, vector_type& z) const
     {
         check(abi<T>::axpbyc(h_, sz_, mul_x, x, mul_y, y, z), "assign_mul_compensated");
+    }
+    // fused Gram-Schmidt helpers: out_j = (V_j, w) for j < k, reading w once;
+    // y := y + sum_j a_j V_j in one pass (compensated in high precision)
+    void multi_scalar_prod(int k, const vector_type* V, const vector_type& w, scalar_type* out) const
+    {
+        T buf[4 * 64];
+        for (int j0 = 0; j0 < k; j0 += 64) {
+            const int kk = (k - j0) < 64 ? (k - j0) : 64;
+            check(abi<T>::multidot(h_, kk, (const T* const*)(V + j0), sz_, w, high_ ? 1 : 0, buf), "multi_scalar_prod");
+            for (int j = 0; j < kk; ++j) out[j0 + j] = buf[4 * j + 2];
+        }
+    }
+    void multi_add_mul(int k, const scalar_type* a, const vector_type* V, vector_type& y) const
+    {
+        for (int j0 = 0; j0 < k; j0 += 64) {
+            const int kk = (k - j0) < 64 ? (k - j0) : 64;
+            check(abi<T>::multiaxpy(h_, kk, a + j0, (const T* const*)(V + j0), sz_, y, high_ ? 1 : 0), "multi_add_mul");
+        }
     }
     void set_value_at_point(scalar_type val_x, size_t at, vector_type& x) const
     {

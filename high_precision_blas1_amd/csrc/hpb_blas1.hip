@@ -232,6 +232,21 @@ __global__ __launch_bounds__(256) void k_convert(size_t n, const S* x, D* y)
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) y[i] = (D)x[i];
 }
 
+// y32 := (float)(s * x64)   (scaled FP64 -> FP32 residual of iterative refinement)
+__global__ __launch_bounds__(256) void k_scale_to_f32(size_t n, double s, const double* __restrict__ x,
+                                                      float* __restrict__ y)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) y[i] = (float)(s * x[i]);
+}
+
+// y64 := fma(a, (double)x32, y64)   (FP64 refinement update from an FP32 correction)
+__global__ __launch_bounds__(256) void k_axpy_f32_f64(size_t n, double a, const float* __restrict__ x,
+                                                      double* __restrict__ y)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        y[i] = __builtin_fma(a, (double)x[i], y[i]);
+}
+
 static int simple_grid(hpb_handle_t h, size_t n)
 {
     size_t g = (n + 255) / 256;
@@ -419,5 +434,23 @@ int hpb_random_uniform_f32(hpb_handle_t h, size_t n, uint64_t seed, uint32_t sid
 
 int hpb_convert_f32_to_f64(hpb_handle_t h, size_t n, const float* x, double* y) { return convert_entry<float, double>(h, n, x, y); }
 int hpb_convert_f64_to_f32(hpb_handle_t h, size_t n, const double* x, float* y) { return convert_entry<double, float>(h, n, x, y); }
+
+int hpb_scale_convert_f64_to_f32(hpb_handle_t h, size_t n, double s, const double* x, float* y)
+{
+    HPB_REQUIRE(h && (n == 0 || (x && y)), HPB_ERR_INVALID, "hpb_scale_convert_f64_to_f32: null argument");
+    if (!n) return HPB_OK;
+    hipLaunchKernelGGL(k_scale_to_f32, dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, s, x, y);
+    HPB_LAUNCH_CHECK();
+    return HPB_OK;
+}
+
+int hpb_axpy_f32_to_f64(hpb_handle_t h, size_t n, double a, const float* x, double* y)
+{
+    HPB_REQUIRE(h && (n == 0 || (x && y)), HPB_ERR_INVALID, "hpb_axpy_f32_to_f64: null argument");
+    if (!n) return HPB_OK;
+    hipLaunchKernelGGL(k_axpy_f32_f64, dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, a, x, y);
+    HPB_LAUNCH_CHECK();
+    return HPB_OK;
+}
 
 }  // extern "C"

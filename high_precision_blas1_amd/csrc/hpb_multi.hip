@@ -1,0 +1,326 @@
+// hpb_multi.hip -- fused multi-vector kernels for Gram-Schmidt.
+//
+// The reference orthogonalises with modified Gram-Schmidt, one dot and one axpy
+// per basis vector, each a separate pass over the vectors and a host round trip
+// (source/numerical_algos/lin_solvers/gmres.h:426-452).  These kernels let a
+// classical Gram-Schmidt pass read w once for k basis vectors:
+//   multidot:  out_j = V_j . w,  j < k <= 64   (compensated or plain)
+//   multiaxpy: y := y + sum_j a_j V_j          (compensated or plain), one pass
+// Algorithmic bytes: multidot (k + 1) n sizeof(T); multiaxpy (k + 2) n sizeof(T).
+//
+// multidot grid: blockIdx.y picks a chunk of KC columns, blockIdx.x strides over
+// tiles of w; every thread keeps KC (s, c) accumulators.  Per-block partials go
+// to the handle workspace; the block taking the last ticket merges each column's
+// G partials in block order (one wave per column: lane l takes partials
+// l, l+64, ...; then the wave butterfly), so results are reproducible.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "hpb_device.hpp"
+#include "hpb_internal.hpp"
+
+namespace hpb {
+
+template <typename T>
+struct ptr_pack {
+    const T* p[HPB_MAX_MULTIDOT];
+};
+template <typename T>
+struct coef_pack {
+    T a[HPB_MAX_MULTIDOT];
+};
+
+template <typename T> struct ubits { using type = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned>; };
+template <typename T>
+__device__ __forceinline__ void st_ag(T* p, T v)
+{
+    using U = typename ubits<T>::type;
+    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_ag(const T* p)
+{
+    using U = typename ubits<T>::type;
+    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <typename T, int KC, bool COMP, int BLOCK, bool VEC>
+__global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T> V, const T* __restrict__ w,
+                                                    T* __restrict__ part, unsigned* __restrict__ ticket,
+                                                    T* __restrict__ out)
+{
+    using Vt = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
+    constexpr int W = VEC ? dev::vec16<T>::W : 1;
+    const unsigned G = gridDim.x;
+    const int j0 = blockIdx.y * KC;
+    const int kc = (k - j0) < KC ? (k - j0) : KC;
+    const size_t nv = n / W;
+    T s[KC], c[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+        s[j] = T(0);
+        c[j] = T(0);
+    }
+    const Vt* wv = reinterpret_cast<const Vt*>(w);
+    for (size_t vi = (size_t)blockIdx.x * BLOCK + threadIdx.x; vi < nv; vi += (size_t)G * BLOCK) {
+        const Vt a = wv[vi];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+            if (j < kc) {
+                const Vt b = reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
+#pragma unroll
+                for (int q = 0; q < W; ++q) {
+                    T av, bv;
+                    if constexpr (VEC) {
+                        av = dev::vget<T>(a, q);
+                        bv = dev::vget<T>(b, q);
+                    } else {
+                        av = a;
+                        bv = b;
+                    }
+                    if (COMP) dev::dot2_step<T>(s[j], c[j], bv, av);
+                    else s[j] = dev::fma_rn<T>(bv, av, s[j]);
+                }
+            }
+        }
+    }
+    if constexpr (VEC) {
+        const size_t r0 = nv * W;
+        if (blockIdx.x == 0 && threadIdx.x < n - r0) {
+            const size_t i = r0 + threadIdx.x;
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) {
+                    if (COMP) dev::dot2_step<T>(s[j], c[j], V.p[j0 + j][i], w[i]);
+                    else s[j] = dev::fma_rn<T>(V.p[j0 + j][i], w[i], s[j]);
+                }
+        }
+    }
+    // per-block totals, column by column
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+        if (j < kc) {
+            T ss = s[j], cc = c[j];
+            dev::block_reduce<T, COMP, BLOCK>(ss, cc);
+            if (threadIdx.x == 0) {
+                st_ag<T>(&part[2 * ((size_t)(j0 + j) * G + blockIdx.x)], ss);
+                st_ag<T>(&part[2 * ((size_t)(j0 + j) * G + blockIdx.x) + 1], cc);
+            }
+        }
+    }
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+        const unsigned total = G * gridDim.y;
+        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == total - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int NW = BLOCK / 64;
+    for (int j = wid; j < k; j += NW) {
+        T fs = T(0), fc = T(0);
+        for (unsigned b = lane; b < G; b += 64) {
+            const T ps = ld_ag<T>(&part[2 * ((size_t)j * G + b)]);
+            const T pc = ld_ag<T>(&part[2 * ((size_t)j * G + b) + 1]);
+            if (COMP) dev::merge<T>(fs, fc, ps, pc);
+            else fs = fs + ps;
+        }
+        dev::wave_reduce<T, COMP>(fs, fc);
+        if (lane == 0) {
+            const T r = fs + fc;
+            out[4 * j] = fs;
+            out[4 * j + 1] = COMP ? fc : T(0);
+            out[4 * j + 2] = COMP ? r : fs;
+            out[4 * j + 3] = COMP ? r : fs;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// y := y + sum_j a_j V_j (compensated: Dot2 over the k+1 terms of each element)
+template <typename T, bool COMP, bool VEC, bool DEVCOEF>
+__global__ __launch_bounds__(256) void k_multiaxpy(size_t n, int k, coef_pack<T> a, const T* __restrict__ da,
+                                                   ptr_pack<T> V, T* __restrict__ y)
+{
+    using Vt = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
+    constexpr int W = VEC ? dev::vec16<T>::W : 1;
+    const size_t nv = n / W;
+    __shared__ T sa[HPB_MAX_MULTIDOT];
+    if (threadIdx.x < (unsigned)k) sa[threadIdx.x] = DEVCOEF ? da[threadIdx.x] : a.a[threadIdx.x];
+    __syncthreads();
+    Vt* yv = reinterpret_cast<Vt*>(y);
+    for (size_t vi = (size_t)blockIdx.x * 256 + threadIdx.x; vi < nv; vi += (size_t)gridDim.x * 256) {
+        Vt yy = yv[vi];
+        T s[W], c[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            if constexpr (VEC) s[q] = dev::vget<T>(yy, q);
+            else s[q] = yy;
+            c[q] = T(0);
+        }
+        for (int j = 0; j < k; ++j) {
+            const Vt b = reinterpret_cast<const Vt*>(V.p[j])[vi];
+            const T aj = sa[j];
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                T bv;
+                if constexpr (VEC) bv = dev::vget<T>(b, q);
+                else bv = b;
+                if (COMP) dev::dot2_step<T>(s[q], c[q], aj, bv);
+                else s[q] = dev::fma_rn<T>(aj, bv, s[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const T r = COMP ? s[q] + c[q] : s[q];
+            if constexpr (VEC) reinterpret_cast<T*>(&yy)[q] = r;
+            else yy = r;
+        }
+        yv[vi] = yy;
+    }
+    if constexpr (VEC) {
+        const size_t r0 = nv * W;
+        if (blockIdx.x == 0 && threadIdx.x < n - r0) {
+            const size_t i = r0 + threadIdx.x;
+            T s = y[i], c = T(0);
+            for (int j = 0; j < k; ++j) {
+                if (COMP) dev::dot2_step<T>(s, c, sa[j], V.p[j][i]);
+                else s = dev::fma_rn<T>(sa[j], V.p[j][i], s);
+            }
+            y[i] = COMP ? s + c : s;
+        }
+    }
+}
+
+static inline bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+template <typename T>
+static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, const T* w, int comp, T* dev_out,
+                          T* host_out)
+{
+    HPB_REQUIRE(h && V && (n == 0 || w), HPB_ERR_INVALID, "hpb_multidot: null argument");
+    HPB_REQUIRE(k >= 1 && k <= HPB_MAX_MULTIDOT, HPB_ERR_INVALID, "hpb_multidot: need 1 <= k <= 64");
+    HPB_REQUIRE(dev_out || host_out, HPB_ERR_INVALID, "hpb_multidot: null output");
+    ptr_pack<T> pk;
+    bool vec = al16(w);
+    for (int j = 0; j < k; ++j) {
+        HPB_REQUIRE(n == 0 || V[j], HPB_ERR_INVALID, "hpb_multidot: null basis vector");
+        pk.p[j] = V[j];
+        vec = vec && al16(V[j]);
+    }
+    for (int j = k; j < HPB_MAX_MULTIDOT; ++j) pk.p[j] = nullptr;
+    constexpr int KC = 8;
+    const int nch = (k + KC - 1) / KC;
+    const size_t W = vec ? dev::vec16<T>::W : 1;
+    const size_t nv = n / W;
+    size_t g = (nv + 255) / 256;
+    const size_t cap = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
+    T* part = reinterpret_cast<T*>(h->part);
+    unsigned* tk = h->ticket + 1;
+    dim3 grid((unsigned)g, (unsigned)nch);
+    if (comp) {
+        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, true, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
+        else hipLaunchKernelGGL((k_multidot<T, KC, true, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
+    } else {
+        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, false, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
+        else hipLaunchKernelGGL((k_multidot<T, KC, false, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
+    }
+    HPB_LAUNCH_CHECK();
+    if (!dev_out) {
+        HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T) * k, hipMemcpyDeviceToHost, h->stream));
+        HPB_HIP(hipStreamSynchronize(h->stream));
+        const T* r = reinterpret_cast<const T*>(h->hres);
+        for (int i = 0; i < 4 * k; ++i) host_out[i] = r[i];
+    }
+    return HPB_OK;
+}
+
+template <typename T>
+static int multiaxpy_entry(hpb_handle_t h, int k, const T* a, const T* dev_a, const T* const* V, size_t n, T* y,
+                           int comp)
+{
+    HPB_REQUIRE(h && V && (a || dev_a) && (n == 0 || y), HPB_ERR_INVALID, "hpb_multiaxpy: null argument");
+    HPB_REQUIRE(k >= 1 && k <= HPB_MAX_MULTIDOT, HPB_ERR_INVALID, "hpb_multiaxpy: need 1 <= k <= 64");
+    if (n == 0) return HPB_OK;
+    ptr_pack<T> pk;
+    coef_pack<T> ca;
+    bool vec = al16(y);
+    for (int j = 0; j < HPB_MAX_MULTIDOT; ++j) {
+        pk.p[j] = j < k ? V[j] : nullptr;
+        ca.a[j] = (j < k && a) ? a[j] : T(0);
+        if (j < k) {
+            HPB_REQUIRE(V[j], HPB_ERR_INVALID, "hpb_multiaxpy: null basis vector");
+            vec = vec && al16(V[j]);
+        }
+    }
+    const size_t W = vec ? dev::vec16<T>::W : 1;
+    size_t g = (n / W + 255) / 256;
+    const size_t cap = (size_t)(h->num_cu > 0 ? h->num_cu : 256) * 8;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    const bool dc = dev_a != nullptr;
+#define HPB_MAXPY(C, VV, D) \
+    hipLaunchKernelGGL((k_multiaxpy<T, C, VV, D>), dim3((unsigned)g), dim3(256), 0, h->stream, n, k, ca, dev_a, pk, y)
+    if (comp) {
+        if (vec) { if (dc) HPB_MAXPY(true, true, true); else HPB_MAXPY(true, true, false); }
+        else { if (dc) HPB_MAXPY(true, false, true); else HPB_MAXPY(true, false, false); }
+    } else {
+        if (vec) { if (dc) HPB_MAXPY(false, true, true); else HPB_MAXPY(false, true, false); }
+        else { if (dc) HPB_MAXPY(false, false, true); else HPB_MAXPY(false, false, false); }
+    }
+#undef HPB_MAXPY
+    HPB_LAUNCH_CHECK();
+    return HPB_OK;
+}
+
+}  // namespace hpb
+
+using namespace hpb;
+
+extern "C" {
+
+int hpb_multidot_f64(hpb_handle_t h, int k, const double* const* V, size_t n, const double* w, int compensated,
+                     double* out4k)
+{ return multidot_entry<double>(h, k, V, n, w, compensated, nullptr, out4k); }
+int hpb_multidot_f32(hpb_handle_t h, int k, const float* const* V, size_t n, const float* w, int compensated,
+                     float* out4k)
+{ return multidot_entry<float>(h, k, V, n, w, compensated, nullptr, out4k); }
+int hpb_multidot_f64_async(hpb_handle_t h, int k, const double* const* V, size_t n, const double* w, int compensated,
+                           double* dev_out4k)
+{
+    HPB_REQUIRE(dev_out4k, HPB_ERR_INVALID, "hpb_multidot_async: null device output");
+    return multidot_entry<double>(h, k, V, n, w, compensated, dev_out4k, nullptr);
+}
+int hpb_multidot_f32_async(hpb_handle_t h, int k, const float* const* V, size_t n, const float* w, int compensated,
+                           float* dev_out4k)
+{
+    HPB_REQUIRE(dev_out4k, HPB_ERR_INVALID, "hpb_multidot_async: null device output");
+    return multidot_entry<float>(h, k, V, n, w, compensated, dev_out4k, nullptr);
+}
+int hpb_multiaxpy_f64(hpb_handle_t h, int k, const double* a, const double* const* V, size_t n, double* y,
+                      int compensated)
+{ return multiaxpy_entry<double>(h, k, a, nullptr, V, n, y, compensated); }
+int hpb_multiaxpy_f32(hpb_handle_t h, int k, const float* a, const float* const* V, size_t n, float* y,
+                      int compensated)
+{ return multiaxpy_entry<float>(h, k, a, nullptr, V, n, y, compensated); }
+int hpb_multiaxpy_dev_f64(hpb_handle_t h, int k, const double* dev_a, const double* const* V, size_t n, double* y,
+                          int compensated)
+{
+    HPB_REQUIRE(dev_a, HPB_ERR_INVALID, "hpb_multiaxpy_dev: null device coefficients");
+    return multiaxpy_entry<double>(h, k, nullptr, dev_a, V, n, y, compensated);
+}
+int hpb_multiaxpy_dev_f32(hpb_handle_t h, int k, const float* dev_a, const float* const* V, size_t n, float* y,
+                          int compensated)
+{
+    HPB_REQUIRE(dev_a, HPB_ERR_INVALID, "hpb_multiaxpy_dev: null device coefficients");
+    return multiaxpy_entry<float>(h, k, nullptr, dev_a, V, n, y, compensated);
+}
+
+}  // extern "C"

@@ -207,6 +207,30 @@ int hpb_random_uniform_f32(hpb_handle_t h, size_t n, uint64_t seed, uint32_t str
 int hpb_convert_f32_to_f64(hpb_handle_t h, size_t n, const float* x, double* y);
 int hpb_convert_f64_to_f32(hpb_handle_t h, size_t n, const double* x, float* y);
 
+/* ---- fused multi-vector kernels (new: the reference's Gram-Schmidt is one
+ *      dot + one axpy per basis vector, gmres.h:426-452) ------------------- */
+/* out4k[4j..4j+3] = {s, c, fl(s+c), fl(s+c)} of V_j . w for j < k <= 64, reading w
+ * once per chunk of 8 columns; V is a HOST array of k device pointers. */
+int hpb_multidot_f64(hpb_handle_t h, int k, const double* const* V, size_t n, const double* w, int compensated,
+                     double* out4k);
+int hpb_multidot_f32(hpb_handle_t h, int k, const float* const* V, size_t n, const float* w, int compensated,
+                     float* out4k);
+int hpb_multidot_f64_async(hpb_handle_t h, int k, const double* const* V, size_t n, const double* w, int compensated,
+                           double* dev_out4k);
+int hpb_multidot_f32_async(hpb_handle_t h, int k, const float* const* V, size_t n, const float* w, int compensated,
+                           float* dev_out4k);
+/* y := y + sum_{j<k} a_j V_j in one pass (k <= 64); compensated evaluates every
+ * element as a Dot2 over its k + 1 terms.  a is a host array (the _dev form
+ * reads the k coefficients from device memory). */
+int hpb_multiaxpy_f64(hpb_handle_t h, int k, const double* a, const double* const* V, size_t n, double* y,
+                      int compensated);
+int hpb_multiaxpy_f32(hpb_handle_t h, int k, const float* a, const float* const* V, size_t n, float* y,
+                      int compensated);
+int hpb_multiaxpy_dev_f64(hpb_handle_t h, int k, const double* dev_a, const double* const* V, size_t n, double* y,
+                          int compensated);
+int hpb_multiaxpy_dev_f32(hpb_handle_t h, int k, const float* dev_a, const float* const* V, size_t n, float* y,
+                          int compensated);
+
 /* ---- CSR sparse matrices (csr::matrix, source/common/csr/matrix.h:91-158:
  *      zero-based int32 row_ptr[rows+1], col_ind[nnz], values[nnz]) ---------- */
 /* y := alpha*A*x + beta*y; beta == 0 does not read y.  Replaces
@@ -250,6 +274,55 @@ int hpb_lognormal_field_f64(hpb_handle_t h, int64_t ncell, double sigma, uint64_
  * of the two cells (the cell's own kappa on boundary faces); off-diagonal -face,
  * diagonal = sum of the four faces ((s + w) + e) + n. */
 int hpb_csr_diffusion2d_f64(hpb_handle_t h, int m, const double* kappa, int* row_ptr, int* col_ind, double* val);
+
+/* ---- Krylov solvers on CSR matrices ---------------------------------------- */
+/* The solver templates (gmres.h:65-495, bicgstabl.h:68-275 of the reference's
+ * numerical_algos/lin_solvers, restated in high_precision_blas1_amd/cpp) over
+ * hip_vector_operations and csr::hip_matrix, unpreconditioned, x0 = x on entry. */
+#define HPB_SOLVER_GMRES 0
+#define HPB_SOLVER_BICGSTABL 1
+typedef struct hpb_solver_params {
+    int solver;              /* HPB_SOLVER_GMRES or HPB_SOLVER_BICGSTABL */
+    int restart;             /* GMRES(m): m, 1 <= m < 256 (gmres.h:355-371) */
+    int basis;               /* BiCGStab(L): L, 1..30 (bicgstabl.h:142-147) */
+    int max_iters, min_iters;
+    double rel_tol, abs_tol; /* monitor: ||r|| <= abs_tol + rel_tol ||b|| (default_monitor.h:159) */
+    int high_precision;      /* 1: compensated Dot2 / nrm2 in the solver (use_high_precision) */
+    int compensated_spmv;    /* 1: compensated-row SpMV inside the solver */
+    int gram_schmidt;        /* GMRES: 0 MGS (the reference's), 1 CGS2 with fused kernels */
+    int early_exit;          /* GMRES: leave a cycle when the Givens estimate meets the tolerance */
+    int reorthogonalization; /* GMRES MGS: one extra correction per basis vector */
+    int resid_recalc_freq;   /* BiCGStab(L): recompute b - Ax every k cycles (0 never) */
+    int save_history;        /* keep (iteration, ||r||/||b||) pairs */
+    int verbose;             /* 0 silent, 1 errors, 2 per-check info */
+} hpb_solver_params;
+typedef struct hpb_solver_result {
+    int converged;           /* monitor converged (true residual for GMRES restarts) */
+    int iters;               /* iterations (inner iterations summed for refinement) */
+    int outer_iters;         /* refinement sweeps (1 for a plain solve) */
+    int breakdown;           /* BiCGStab(L): rho = 0 */
+    int history_len;
+    double resid_norm;       /* last residual norm seen by the monitor */
+    double rhs_norm;
+    double true_rel_resid;   /* ||b - A x|| / ||b||, compensated FP64, after the solve */
+    double seconds;          /* wall time of the solve (stream synchronised at both ends) */
+} hpb_solver_result;
+void hpb_solver_params_default(hpb_solver_params* p);
+/* FP64 solve.  history (host, optional) receives up to history_cap pairs. */
+int hpb_csr_solve_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
+                      const double* val, const double* b, double* x, const hpb_solver_params* p,
+                      hpb_solver_result* res, double* history, int history_cap);
+/* Mixed precision (new): FP32 copies of the values and FP32 Krylov vectors with
+ * compensated FP32 dots inside, FP64 iterative refinement outside:
+ *   r = b - A x (compensated FP64 rows); stop when ||r|| <= rel_tol ||b||;
+ *   solve A d = r/||r|| in FP32 with `inner` (its rel_tol is the inner tolerance);
+ *   x += ||r|| d in FP64. */
+int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
+                         const double* val, const double* b, double* x, const hpb_solver_params* inner,
+                         double rel_tol, int max_outer, hpb_solver_result* res);
+/* helpers of the refinement: y32 := (float)(s x64); y64 := fma(a, (double)x32, y64) */
+int hpb_scale_convert_f64_to_f32(hpb_handle_t h, size_t n, double s, const double* x, float* y);
+int hpb_axpy_f32_to_f64(hpb_handle_t h, size_t n, double a, const float* x, double* y);
 
 /* ---- ill-conditioned pair generator -------------------------------------- */
 /* Device-side, seeded GenDot pair (Ogita-Rump-Oishi 2005 Alg. 6.1) replacing

@@ -1,0 +1,129 @@
+"""GPU Krylov solvers through the C ABI (hpb_csr_solve_f64 / hpb_csr_solve_ir_f64).
+
+* Small convection-diffusion problems: every variant reaches the true relative
+  residual it reports (recomputed here on the host) and agrees with scipy's
+  direct solve.
+* Parity with the reference algorithm: GMRES(m)/MGS and BiCGStab(L) restated in
+  numpy (oracle/krylov.py, pinned against scipy) need the same number of
+  iterations within one restart cycle / one BiCGStab(L) cycle.
+* Mixed precision: FP32 inner solves inside FP64 refinement reach 1e-12.
+* Failure paths forced from the host: iteration cap, NaN right-hand side.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import high_precision_blas1_amd as H
+import high_precision_blas1_amd.solvers as S
+import oracle as O
+from oracle import krylov as K
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(handle, nx, ny, nz, coef, seed=2):
+    Mh = O.stencil7(nx, ny, nz, coef)
+    A = S.DeviceCSR.stencil7(handle, nx, ny, nz, coef)
+    r = np.random.default_rng(seed)
+    xs = r.uniform(-1, 1, Mh.shape[0])
+    b = Mh @ xs
+    return Mh, A, b
+
+
+def _host_rel(Mh, b, x):
+    return np.linalg.norm(b - Mh @ x) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("gs", [0, 1])
+@pytest.mark.parametrize("high", [0, 1])
+def test_gmres_small(handle, gs, high):
+    Mh, A, b = _problem(handle, 16, 14, 12, S.CONVDIFF7)
+    say("gmres", gs, high)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    p = S.params(solver=S.GMRES, restart=30, rel_tol=1e-12, max_iters=3000, high_precision=high, gram_schmidt=gs,
+                 early_exit=0)
+    res = S.solve(handle, A, db, dx, p)
+    x = dx.numpy()
+    assert res["converged"] == 1
+    assert res["true_rel_resid"] <= 1.5e-12
+    assert abs(_host_rel(Mh, b, x) - res["true_rel_resid"]) <= 1e-13
+    xd = spla.spsolve(Mh.tocsc(), b)
+    assert np.max(np.abs(x - xd)) < 1e-9
+    xo, ito, hist = K.gmres_mgs(Mh, b, np.zeros_like(b), 30, 1e-12, 3000)
+    assert abs(res["iters"] - ito) <= 30, (res["iters"], ito)
+    A.free()
+
+
+@pytest.mark.parametrize("L", [1, 2, 4])
+def test_bicgstabl_small(handle, L):
+    Mh, A, b = _problem(handle, 16, 14, 12, S.CONVDIFF7)
+    say("bicgstabl", L)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    p = S.params(solver=S.BICGSTABL, basis=L, rel_tol=1e-12, max_iters=5000, high_precision=1, resid_recalc_freq=1)
+    res = S.solve(handle, A, db, dx, p)
+    assert res["converged"] == 1 and res["breakdown"] == 0
+    x = dx.numpy()
+    assert _host_rel(Mh, b, x) <= 1.5e-12
+    xo, ito, hist = K.bicgstabl(Mh, b, np.zeros_like(b), L, 1e-12, 5000)
+    assert abs(res["iters"] - ito) <= 2 * L + 4, (res["iters"], ito)
+    A.free()
+
+
+@pytest.mark.parametrize("inner_solver,arg", [(S.GMRES, 30), (S.BICGSTABL, 2)])
+def test_mixed_precision_refinement(handle, inner_solver, arg):
+    Mh, A, b = _problem(handle, 24, 24, 24, S.CONVDIFF7)
+    say("ir", inner_solver, arg)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    inner = S.params(solver=inner_solver, restart=arg, basis=arg, rel_tol=1e-5, max_iters=2000, high_precision=1,
+                     early_exit=1, gram_schmidt=1)
+    res = S.solve_ir(handle, A, db, dx, inner, rel_tol=1e-12, max_outer=30)
+    assert res["converged"] == 1
+    assert res["true_rel_resid"] <= 1e-12
+    assert _host_rel(Mh, b, dx.numpy()) <= 1.1e-12
+    assert 1 <= res["outer_iters"] <= 10
+    A.free()
+
+
+def test_failure_paths(handle):
+    Mh, A, b = _problem(handle, 10, 10, 10, S.CONVDIFF7)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    res = S.solve(handle, A, db, dx, S.params(solver=S.GMRES, restart=5, rel_tol=1e-14, max_iters=10))
+    assert res["converged"] == 0 and res["iters"] >= 10
+    bb = b.copy()
+    bb[3] = np.nan
+    db.upload(bb)
+    dx.upload(np.zeros_like(b))
+    res = S.solve(handle, A, db, dx, S.params(solver=S.BICGSTABL, basis=2, rel_tol=1e-10, max_iters=100))
+    assert res["converged"] == 0
+    with pytest.raises(H.HPBError):
+        S.solve(handle, A, db, dx, S.params(solver=S.GMRES, restart=0))
+    A.free()
+
+
+def test_convdiff_128_gmres_and_bicgstab(handle):
+    # the FP64 Krylov workload at its full size: 128^3 upwind convection-diffusion
+    say("128^3 convdiff")
+    A = S.DeviceCSR.stencil7(handle, 128, 128, 128, S.CONVDIFF7)
+    n = A.rows
+    xs = H.DeviceArray(n, np.float64, handle)
+    H.call("hpb_random_uniform_f64", handle.ptr, n, 2, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), xs.ptr)
+    db = H.DeviceArray(n, np.float64, handle)
+    A.spmv(xs, db, compensated=True)
+    for p in (S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000, gram_schmidt=1, early_exit=1),
+              S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000)):
+        dx = H.DeviceArray(n, np.float64, handle)
+        H.call("hpb_fill_f64", handle.ptr, n, ctypes.c_double(0.0), dx.ptr)
+        res = S.solve(handle, A, db, dx, p)
+        say(res)
+        assert res["converged"] == 1 and res["true_rel_resid"] <= 1.5e-12
+        err = np.max(np.abs(dx.numpy() - xs.numpy()))
+        assert err < 1e-8
+        dx.free()
+    A.free()
