@@ -67,9 +67,11 @@ class Handle:
              ctypes.byref(b), ctypes.byref(u), ctypes.byref(g), ctypes.byref(f))
         return b.value, u.value, g.value, f.value
 
-    def reduce_config(self, dtype, n, aligned=True):
+    def reduce_config(self, dtype, n, aligned=True, binary=True):
+        """(block, unroll, grid, vector width) of a reduction; binary: two-input dot."""
         cfg = (ctypes.c_int * 4)()
-        call("hpb_reduce_config", self._h, 0 if np.dtype(dtype) == np.float64 else 1, n, int(aligned), cfg)
+        call("hpb_reduce_config", self._h, 0 if np.dtype(dtype) == np.float64 else 1, n, int(aligned), int(binary),
+             cfg)
         return tuple(cfg)
 
     def timer_start(self):

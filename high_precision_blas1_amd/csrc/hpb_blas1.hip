@@ -149,6 +149,7 @@ template <typename T, class F, int NIN>
 static int map_entry(hpb_handle_t h, size_t n, const F& f, const T* i0, const T* i1, const T* i2, const T* i3,
                      T* out)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h, HPB_ERR_INVALID, "elementwise: null handle");
     if (n == 0) return HPB_OK;
     HPB_REQUIRE(out, HPB_ERR_INVALID, "elementwise: null output");
@@ -259,6 +260,7 @@ static int simple_grid(hpb_handle_t h, size_t n)
 template <typename T>
 static int valid_entry(hpb_handle_t h, size_t n, const T* x, int* out)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && out && (n == 0 || x), HPB_ERR_INVALID, "hpb_valid: null argument");
     hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, h->stream, h->dflag);
     if (n) hipLaunchKernelGGL((k_valid<T>), dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, x, h->dflag);
@@ -274,6 +276,7 @@ static int valid_entry(hpb_handle_t h, size_t n, const T* x, int* out)
 template <typename T>
 static int set_value_entry(hpb_handle_t h, size_t n, size_t at, T val, T* x)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && x, HPB_ERR_INVALID, "hpb_set_value: null argument");
     HPB_REQUIRE(at < n, HPB_ERR_INVALID, "hpb_set_value: index out of range");
     hipLaunchKernelGGL((k_set_value<T>), dim3(1), dim3(1), 0, h->stream, x, at, val);
@@ -295,6 +298,7 @@ static int get_value_entry(hpb_handle_t h, size_t n, size_t at, const T* x, T* o
 template <typename T>
 static int random_entry(hpb_handle_t h, size_t n, uint64_t seed, uint32_t sid, T a, T b, T* x)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && (n == 0 || x), HPB_ERR_INVALID, "hpb_random_uniform: null argument");
     if (!n) return HPB_OK;
     hipLaunchKernelGGL((k_random<T>), dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, seed, sid, a, b, x);
@@ -305,6 +309,7 @@ static int random_entry(hpb_handle_t h, size_t n, uint64_t seed, uint32_t sid, T
 template <typename T>
 static int swap_entry(hpb_handle_t h, size_t n, T* x, T* y)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && (n == 0 || (x && y)), HPB_ERR_INVALID, "hpb_swap: null argument");
     if (!n) return HPB_OK;
     hipLaunchKernelGGL((k_swap<T>), dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, x, y);
@@ -315,6 +320,7 @@ static int swap_entry(hpb_handle_t h, size_t n, T* x, T* y)
 template <typename S, typename D>
 static int convert_entry(hpb_handle_t h, size_t n, const S* x, D* y)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && (n == 0 || (x && y)), HPB_ERR_INVALID, "hpb_convert: null argument");
     if (!n) return HPB_OK;
     hipLaunchKernelGGL((k_convert<S, D>), dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, x, y);
@@ -437,6 +443,7 @@ int hpb_convert_f64_to_f32(hpb_handle_t h, size_t n, const double* x, float* y) 
 
 int hpb_scale_convert_f64_to_f32(hpb_handle_t h, size_t n, double s, const double* x, float* y)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && (n == 0 || (x && y)), HPB_ERR_INVALID, "hpb_scale_convert_f64_to_f32: null argument");
     if (!n) return HPB_OK;
     hipLaunchKernelGGL(k_scale_to_f32, dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, s, x, y);
@@ -446,6 +453,7 @@ int hpb_scale_convert_f64_to_f32(hpb_handle_t h, size_t n, double s, const doubl
 
 int hpb_axpy_f32_to_f64(hpb_handle_t h, size_t n, double a, const float* x, double* y)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && (n == 0 || (x && y)), HPB_ERR_INVALID, "hpb_axpy_f32_to_f64: null argument");
     if (!n) return HPB_OK;
     hipLaunchKernelGGL(k_axpy_f32_f64, dim3(simple_grid(h, n)), dim3(256), 0, h->stream, n, a, x, y);

@@ -35,13 +35,16 @@ const char* hpb_version(void) { return "hpb 0.1 gfx950"; }
 
 static void default_tuning(hpb_handle_s* h)
 {
-    // Reductions: 256 threads, 4 x 16-byte loads per input per tile, one block
-    // per CU (measured fastest at N = 2^27 on MI355X: more resident blocks lower
-    // the achieved bandwidth).  Elementwise: 256 threads, 2 vectors per thread.
+    // Shapes measured fastest on MI355X at N = 2^27 (FP64): two-input reductions
+    // 512 threads x 4 16-byte loads per input, 3 blocks per 4 CUs, nontemporal
+    // loads; one-input reductions the same with one block per CU; elementwise
+    // 256 threads x 2 vectors, 2 blocks per CU.  More resident blocks lower the
+    // achieved bandwidth of these streams.
     const int cu = h->num_cu > 0 ? h->num_cu : 256;
     for (int d = 0; d < 2; ++d) {
-        h->tune_red[d] = {256, 4, cu, 0};
-        h->tune_map[d] = {256, 2, 8 * cu, 0};
+        h->tune_red[d] = {512, 4, (3 * cu) / 4 > 0 ? (3 * cu) / 4 : 1, HPB_TUNE_NONTEMPORAL};
+        h->tune_red1[d] = {512, 4, cu, HPB_TUNE_NONTEMPORAL};
+        h->tune_map[d] = {256, 2, 2 * cu, 0};
     }
 }
 
@@ -167,7 +170,7 @@ int hpb_memcpy_d2d(hpb_handle_t h, void* dst, const void* src, size_t bytes)
 int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max, int flags)
 {
     HPB_REQUIRE(h, HPB_ERR_INVALID, "hpb_set_tuning: null handle");
-    HPB_REQUIRE(kind == 0 || kind == 1, HPB_ERR_INVALID, "hpb_set_tuning: kind must be 0 or 1");
+    HPB_REQUIRE(kind >= 0 && kind <= 2, HPB_ERR_INVALID, "hpb_set_tuning: kind must be 0, 1 or 2");
     HPB_REQUIRE(dtype == 0 || dtype == 1, HPB_ERR_INVALID, "hpb_set_tuning: dtype must be 0 or 1");
     HPB_REQUIRE(block == 256 || block == 512, HPB_ERR_UNSUPPORTED, "hpb_set_tuning: block must be 256 or 512");
     HPB_REQUIRE(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, HPB_ERR_UNSUPPORTED,
@@ -176,6 +179,7 @@ int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, i
     HPB_REQUIRE((flags & ~HPB_TUNE_NONTEMPORAL) == 0, HPB_ERR_UNSUPPORTED, "hpb_set_tuning: unknown flags");
     hpb_tuning_t t = {block, unroll, grid_max, flags};
     if (kind == 0) h->tune_red[dtype] = t;
+    else if (kind == 2) h->tune_red1[dtype] = t;
     else h->tune_map[dtype] = t;
     return HPB_OK;
 }
@@ -183,8 +187,8 @@ int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, i
 int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max, int* flags)
 {
     HPB_REQUIRE(h && block && unroll && grid_max, HPB_ERR_INVALID, "hpb_get_tuning: null argument");
-    HPB_REQUIRE((kind == 0 || kind == 1) && (dtype == 0 || dtype == 1), HPB_ERR_INVALID, "hpb_get_tuning: bad kind/dtype");
-    const hpb_tuning_t& t = kind == 0 ? h->tune_red[dtype] : h->tune_map[dtype];
+    HPB_REQUIRE((kind >= 0 && kind <= 2) && (dtype == 0 || dtype == 1), HPB_ERR_INVALID, "hpb_get_tuning: bad kind/dtype");
+    const hpb_tuning_t& t = kind == 0 ? h->tune_red[dtype] : (kind == 2 ? h->tune_red1[dtype] : h->tune_map[dtype]);
     *block = t.block;
     *unroll = t.unroll;
     *grid_max = t.grid_max;

@@ -88,6 +88,7 @@ template <typename T, bool COMP, int MODE>
 static int spmv_entry(hpb_handle_t h, int rows, int cols, long nnz, const int* rp, const int* col, const T* val,
                       const T* x, T alpha, T beta, const T* b, T* y, int lpr)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h, HPB_ERR_INVALID, "csr spmv: null handle");
     HPB_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, HPB_ERR_INVALID, "csr spmv: negative size");
     if (rows == 0) return HPB_OK;
@@ -251,6 +252,7 @@ int hpb_stencil_nnz(int nx, int ny, int nz, int64_t* out_rows, int64_t* out_nnz)
 int hpb_csr_stencil7_f64(hpb_handle_t h, int nx, int ny, int nz, const double* coef7, int* row_ptr, int* col_ind,
                          double* val)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && coef7 && row_ptr && col_ind && val, HPB_ERR_INVALID, "hpb_csr_stencil7_f64: null argument");
     int64_t n, nnz;
     int st = hpb_stencil_nnz(nx, ny, nz, &n, &nnz);
@@ -267,6 +269,7 @@ int hpb_csr_stencil7_f64(hpb_handle_t h, int nx, int ny, int nz, const double* c
 
 int hpb_lognormal_field_f64(hpb_handle_t h, int64_t ncell, double sigma, uint64_t seed, double* kappa)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && kappa && ncell > 0, HPB_ERR_INVALID, "hpb_lognormal_field_f64: bad argument");
     hipLaunchKernelGGL(k_lognormal_field, dim3(fill_grid(h, ncell)), dim3(256), 0, h->stream, ncell, sigma, seed, kappa);
     HPB_LAUNCH_CHECK();
@@ -275,6 +278,7 @@ int hpb_lognormal_field_f64(hpb_handle_t h, int64_t ncell, double sigma, uint64_
 
 int hpb_csr_diffusion2d_f64(hpb_handle_t h, int m, const double* kappa, int* row_ptr, int* col_ind, double* val)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && kappa && row_ptr && col_ind && val && m > 0, HPB_ERR_INVALID, "hpb_csr_diffusion2d_f64: bad argument");
     int64_t n, nnz;
     int st = hpb_stencil_nnz(m, m, 1, &n, &nnz);

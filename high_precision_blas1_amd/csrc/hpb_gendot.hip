@@ -251,6 +251,7 @@ using namespace hpb;
 extern "C" int hpb_gendot_f64(hpb_handle_t h, size_t n, double log2cond, uint64_t seed, size_t n_tail, double* x,
                               double* y, uint64_t* out_exact4, double* out3)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && x && y, HPB_ERR_INVALID, "hpb_gendot_f64: null argument");
     HPB_REQUIRE(n >= 2 && n_tail >= 1 && n_tail < n, HPB_ERR_INVALID, "hpb_gendot_f64: need n >= 2, 1 <= n_tail < n");
     const double hb = 0.5 * log2cond;

@@ -29,7 +29,8 @@ struct hpb_handle_s {
     void* hres = nullptr;           // pinned host mirror of dres
     int* dflag = nullptr;           // validity flag
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hpb_tuning_t tune_red[2];       // [0] f64, [1] f32 reductions
+    hpb_tuning_t tune_red[2];       // two-input reductions (dot), [0] f64, [1] f32
+    hpb_tuning_t tune_red1[2];      // one-input reductions (sum, asum, nrm2)
     hpb_tuning_t tune_map[2];       // elementwise kernels
 };
 
@@ -58,6 +59,10 @@ int hip_status(hipError_t e, const char* file, int line, const char* what);
             return (code);                                                       \
         }                                                                        \
     } while (0)
+
+// Entry points that launch kernels first drop a stale error left by an earlier,
+// unrelated runtime call, so that the check after the launch reports the launch.
+#define HPB_CLEAR_STALE() (void)hipGetLastError()
 
 #define HPB_LAUNCH_CHECK()                                                       \
     do {                                                                         \

@@ -202,6 +202,7 @@ template <typename T>
 static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, const T* w, int comp, T* dev_out,
                           T* host_out)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && V && (n == 0 || w), HPB_ERR_INVALID, "hpb_multidot: null argument");
     HPB_REQUIRE(k >= 1 && k <= HPB_MAX_MULTIDOT, HPB_ERR_INVALID, "hpb_multidot: need 1 <= k <= 64");
     HPB_REQUIRE(dev_out || host_out, HPB_ERR_INVALID, "hpb_multidot: null output");
@@ -246,6 +247,7 @@ template <typename T>
 static int multiaxpy_entry(hpb_handle_t h, int k, const T* a, const T* dev_a, const T* const* V, size_t n, T* y,
                            int comp)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h && V && (a || dev_a) && (n == 0 || y), HPB_ERR_INVALID, "hpb_multiaxpy: null argument");
     HPB_REQUIRE(k >= 1 && k <= HPB_MAX_MULTIDOT, HPB_ERR_INVALID, "hpb_multiaxpy: need 1 <= k <= 64");
     if (n == 0) return HPB_OK;

@@ -234,9 +234,9 @@ struct red_shape {
 };
 
 template <typename T>
-static red_shape reduce_shape(const hpb_handle_s* h, size_t n, bool aligned)
+static red_shape reduce_shape(const hpb_handle_s* h, size_t n, bool aligned, bool binary)
 {
-    const hpb_tuning_t& t = h->tune_red[dtype_id<T>::v];
+    const hpb_tuning_t& t = binary ? h->tune_red[dtype_id<T>::v] : h->tune_red1[dtype_id<T>::v];
     red_shape r;
     r.block = t.block;
     r.unroll = t.unroll;
@@ -288,12 +288,13 @@ static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; 
 template <typename T, int OP>
 static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev_out, T* host_out)
 {
+    HPB_CLEAR_STALE();
     HPB_REQUIRE(h, HPB_ERR_INVALID, "reduction: null handle");
     HPB_REQUIRE(n == 0 || x, HPB_ERR_INVALID, "reduction: null x");
     HPB_REQUIRE(!red_traits<OP>::binary || n == 0 || y, HPB_ERR_INVALID, "reduction: null y");
     HPB_REQUIRE(dev_out || host_out, HPB_ERR_INVALID, "reduction: null output");
     const bool al = aligned16(x) && (!red_traits<OP>::binary || aligned16(y));
-    red_shape sh = reduce_shape<T>(h, n, al);
+    red_shape sh = reduce_shape<T>(h, n, al, red_traits<OP>::binary);
     T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
     launch_red<T, OP>(sh, h->stream, n, x, y, reinterpret_cast<T*>(h->part), h->ticket, out);
     HPB_LAUNCH_CHECK();
@@ -312,11 +313,12 @@ using namespace hpb;
 
 extern "C" {
 
-int hpb_reduce_config(hpb_handle_t h, int dtype, size_t n, int aligned, int* cfg4)
+int hpb_reduce_config(hpb_handle_t h, int dtype, size_t n, int aligned, int binary, int* cfg4)
 {
     HPB_REQUIRE(h && cfg4, HPB_ERR_INVALID, "hpb_reduce_config: null argument");
     HPB_REQUIRE(dtype == 0 || dtype == 1, HPB_ERR_INVALID, "hpb_reduce_config: dtype must be 0 or 1");
-    red_shape s = dtype == 0 ? reduce_shape<double>(h, n, aligned != 0) : reduce_shape<float>(h, n, aligned != 0);
+    red_shape s = dtype == 0 ? reduce_shape<double>(h, n, aligned != 0, binary != 0)
+                             : reduce_shape<float>(h, n, aligned != 0, binary != 0);
     cfg4[0] = s.block;
     cfg4[1] = s.unroll;
     cfg4[2] = s.grid;

@@ -189,9 +189,14 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
         A64.adopt_device(rows, rows, nnz, (int*)row_ptr, (int*)col_ind, (double*)val, false);
         float* val32 = nullptr;
         hpb::check(hpb_malloc((void**)&val32, sizeof(float) * (size_t)nnz), "ir: matrix values");
+        struct dev_guard {
+            void* p;
+            ~dev_guard() { hpb_free(p); }
+        } val32_guard{val32};
         hpb::check(hpb_convert_f64_to_f32(h, (size_t)nnz, val, val32), "ir: convert matrix");
+        // FP32 values; the index arrays stay the caller's (not owned)
         s32::mat_t A32(h);
-        A32.adopt_device(rows, rows, nnz, (int*)row_ptr, (int*)col_ind, val32, true);
+        A32.adopt_device(rows, rows, nnz, (int*)row_ptr, (int*)col_ind, val32, false);
         A32.use_compensated_rows(inner->compensated_spmv != 0);
         s32::op_t op32(&vo32, &A32);
         double* r = nullptr;

@@ -69,15 +69,17 @@ int hpb_memcpy_d2h(hpb_handle_t h, void* host_dst, const void* dev_src, size_t b
 int hpb_memcpy_d2d(hpb_handle_t h, void* dev_dst, const void* dev_src, size_t bytes);
 
 /* ---- tuning / introspection -------------------------------------------- */
-/* kind: 0 reductions, 1 elementwise; dtype: 0 f64, 1 f32.  The autotuner
+/* kind: 0 two-input reductions (dot), 1 elementwise, 2 one-input reductions
+ * (sum, asum, nrm2); dtype: 0 f64, 1 f32.  The autotuner
  * (high_precision_blas1_amd/autotune.py) writes these; BLOCK_SIZE=1024 is a fixed
  * template parameter in the reference (gpu_reduction_ogita.h:36). */
 #define HPB_TUNE_NONTEMPORAL 1 /* reductions: nontemporal loads for the streamed tiles */
 int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max, int flags);
 int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max, int* flags);
-/* Launch shape a reduction of n elements would use: cfg = {block, unroll, grid,
+/* Launch shape a reduction of n elements would use (binary: two-input dot, else
+ * one-input sum/asum/nrm2): cfg = {block, unroll, grid,
  * vector width (1 if the pointers are not 16-byte aligned)}. */
-int hpb_reduce_config(hpb_handle_t h, int dtype, size_t n, int aligned16, int* cfg4);
+int hpb_reduce_config(hpb_handle_t h, int dtype, size_t n, int aligned16, int binary, int* cfg4);
 
 /* ---- timing (HIP events on the handle's stream) ------------------------- */
 int hpb_timer_start(hpb_handle_t h);

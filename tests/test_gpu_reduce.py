@@ -41,7 +41,7 @@ def _check_op(handle, op, x, y, offset):
     dy = H.DeviceArray.from_numpy(y, handle, offset_bytes=offset) if binary else None
     gpu = H.reduce(op, handle, dx, dy)
     aligned = (dx.addr % 16 == 0) and (dy is None or dy.addr % 16 == 0)
-    cfg = handle.reduce_config(x.dtype, n, aligned)
+    cfg = handle.reduce_config(x.dtype, n, aligned, binary)
     emu = O.hpb_order_reduce(op, x, y if binary else None, cfg)
     assert gpu[0].tobytes() == emu[0].tobytes(), (op, n, cfg, gpu, emu)
     assert gpu[1].tobytes() == emu[1].tobytes(), (op, n, cfg, gpu, emu)
@@ -67,18 +67,21 @@ def test_bitexact_vs_blocked_order(handle, dtype, op):
 def test_bitexact_all_launch_shapes(handle, dtype):
     n = (1 << 18) + 5
     x, y = _data(n, dtype, 99)
-    saved = handle.get_tuning(0, dtype)
+    saved = [handle.get_tuning(k, dtype) for k in (0, 2)]
     try:
         for block in (256, 512):
             for unroll in (1, 2, 4, 8):
                 for gmax in (1, 3, 64, 1024, 4096):
                     for flags in (0, 1):
                         handle.set_tuning(0, dtype, block, unroll, gmax, flags)
+                        handle.set_tuning(2, dtype, block, unroll, gmax, flags)
                         say("shape", block, unroll, gmax, flags, np.dtype(dtype).name)
                         _check_op(handle, "dot2", x, y, 0)
                         _check_op(handle, "sum2", x, y, 0)
+                        _check_op(handle, "nrm2c", x, y, 0)
     finally:
-        handle.set_tuning(0, dtype, *saved)
+        handle.set_tuning(0, dtype, *saved[0])
+        handle.set_tuning(2, dtype, *saved[1])
 
 
 def _oro_bound(ex, x, y):
