@@ -1,19 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark of the compensated BLAS1 hot path on one MI355X.
+"""Benchmark of the compensated BLAS1 + Krylov path on one MI355X.
 
-Default workload (one "step" = one pass of the hot path over one batch):
-  FP64 Ogita-Rump-Oishi Dot2 of two resident vectors of N = 2^27 elements
-  (2 GiB read per step, far above the 256 MB MALL), GenDot-style ill-conditioned
-  pair generated on the device (kappa 1e16, seed 0).
+Headline (value): FP64 Ogita-Rump-Oishi Dot2 of two device-resident vectors of
+N = 2^27 elements (2 GiB read per step, far above the 256 MB MALL).  The inputs
+are a GenDot pair built on the device (kappa 1e16, seed 0), whose exact dot is
+known.  One "step" = one Dot2 launch over the pair.  The GB/s figure is
+algorithmic bytes (16 B per element) divided by the kernel time, measured with
+HIP events on the handle's stream over K back-to-back launches.
 
-Printed: one JSON line with the whole-job throughput (GB/s of algorithmic bytes,
-16 B per element), the roofline block of the Dot2 kernel (achieved vs the
-8 TB/s HBM3E peak, timed with HIP events on the handle's stream), and the CPU
-baseline: the reference's own CPU Dot2 (oracle/_ref, compiled from
-source/common/threaded_reduction.h and dot_product.h) timed on this host on a
-bounded sample of the same vectors.
+The same JSON line carries:
+  roofline      the Dot2 kernel against the 8 TB/s HBM3E peak;
+  cpu_baseline  the reference's own CPU Dot2 (oracle/_ref, compiled from
+                source/common/threaded_reduction.h) on all host cores, on a
+                2^25-element sample of the same vectors;
+  configs       the other benchmark configurations, measured in the same run:
+    cpu_plain      N = 2^20, kappa 1e16: reference CPU Dot2 / naive / Sum2, errors and times
+    cond_sweep     kappa 1e4..1e32 at N = 2^27: Dot2 GB/s and relative errors (Dot2, naive)
+    blas1          Sum2, compensated nrm2, axpy, compensated axpby GB/s at N = 2^27
+    spmv           256^3 7-point Poisson, plain and compensated rows, GB/s
+    krylov         128^3 upwind convection-diffusion (Peclet 100): GMRES(50), BiCGStab(4),
+                   naive vs compensated dots, to 1e-12 true relative residual
+    mixed          the same matrix: FP32-inner refinement vs plain FP64, time to 1e-12
+    multidot       fused compensated multi-dot, k = 1..64, GB/s
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--headline-only]
 """
 import argparse
 import ctypes
@@ -21,6 +31,7 @@ import json
 import os
 import sys
 import time
+from fractions import Fraction
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -30,41 +41,46 @@ METRIC = "Dot2 GB/s & %HBM peak; Krylov time-to-1e-12, mixed+refined vs FP64 pla
 
 
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print("bench:", *a, file=sys.stderr, flush=True)
 
 
-def make_inputs(H, h, n, cond, seed):
-    """Device vectors x, y of length n; GenDot pair when the generator exists."""
-    x = H.DeviceArray(n, "float64", h)
-    y = H.DeviceArray(n, "float64", h)
-    lib = H.lib()
-    if hasattr(lib, "hpb_gendot_f64"):
-        import numpy as np
-        limbs = np.zeros(4, dtype=np.uint64)
-        dd = np.zeros(3)
-        H.call("hpb_gendot_f64", h.ptr, n, ctypes.c_double(float(np.log2(cond))), seed, n // 2, x.ptr, y.ptr,
-               limbs.ctypes.data_as(ctypes.c_void_p), dd.ctypes.data_as(ctypes.c_void_p))
-        return x, y, {"generator": "gendot", "cond": cond, "achieved_cond": float(dd[2]), "exact_dot": float(dd[0])}
-    H.call("hpb_random_uniform_f64", h.ptr, n, seed, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), x.ptr)
-    H.call("hpb_random_uniform_f64", h.ptr, n, seed, 1, ctypes.c_double(-1.0), ctypes.c_double(1.0), y.ptr)
-    return x, y, {"generator": "uniform(-1,1)"}
+def exact_value(limbs):
+    """Exact dot from the generator's 256-bit fixed-point limbs (LSB 2^-106)."""
+    v = 0
+    for i in range(4):
+        v |= int(limbs[i]) << (64 * i)
+    if v >> 255:
+        v -= 1 << 256
+    return Fraction(v, 1 << 106)
 
 
-def time_dot2(H, h, x, y, n, steps, warmup):
-    dout = H.DeviceArray(4, "float64", h)
-    fn = H.lib().hpb_dot2_f64_async
+def rel_err(res, exact):
+    if exact == 0:
+        return float("inf") if res != 0 else 0.0
+    e = abs(Fraction(float(res)) - exact) / abs(exact)
+    return min(float(e), 1.0)
+
+
+def gendot(H, h, n, cond, seed, x, y):
+    import numpy as np
+    limbs = np.zeros(4, dtype=np.uint64)
+    dd = np.zeros(3)
+    H.call("hpb_gendot_f64", h.ptr, n, ctypes.c_double(float(np.log2(cond))), seed, n // 2, x.ptr, y.ptr,
+           limbs.ctypes.data_as(ctypes.c_void_p), dd.ctypes.data_as(ctypes.c_void_p))
+    return exact_value(limbs), dd
+
+
+def timed(h, fn, args, steps, warmup):
     for _ in range(warmup):
-        rc = fn(h.ptr, n, x.ptr, y.ptr, dout.ptr)
+        rc = fn(*args)
         if rc:
-            raise RuntimeError(H.lib().hpb_last_error().decode())
+            from high_precision_blas1_amd import lib
+            raise RuntimeError(lib().hpb_last_error().decode())
     h.sync()
     h.timer_start()
     for _ in range(steps):
-        fn(h.ptr, n, x.ptr, y.ptr, dout.ptr)
-    ms = h.timer_stop()
-    res = dout.numpy()
-    dout.free()
-    return ms, res
+        fn(*args)
+    return h.timer_stop() / steps
 
 
 def cpu_baseline(H, h, x, y, n, budget_s=12.0):
@@ -87,7 +103,7 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
         fthr = lambda: O.lib().orc_dot2_threaded(sample, O.ptr(hx), O.ptr(hy), cores)
         fseq = lambda: O.dot2(hx, hy)
 
-    def timed(f, budget):
+    def run(f, budget):
         ts = []
         t_end = time.perf_counter() + budget
         while len(ts) < 4 or (time.perf_counter() < t_end and len(ts) < 200):
@@ -98,8 +114,8 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
         core = ts[1:-1]  # drop min and max (test_vector_operations.cpp:44-49)
         return sum(core) / len(core), len(ts)
 
-    t_thr, reps_thr = timed(fthr, budget_s * 0.6)
-    t_seq, reps_seq = timed(fseq, budget_s * 0.4)
+    t_thr, reps_thr = run(fthr, budget_s * 0.6)
+    t_seq, reps_seq = run(fseq, budget_s * 0.4)
     nbytes = 16.0 * sample
     return {
         "value": round(nbytes / t_thr / 1e9, 3),
@@ -113,6 +129,229 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
     }
 
 
+def cfg_cpu_plain():
+    """N = 2^20, kappa 1e16, seed 0 on the host: the reference's CPU Dot2, naive dot,
+    threaded Dot2 and Sum2 of the 2N exact product terms, relative errors and times."""
+    import numpy as np
+    import oracle as O
+    n = 1 << 20
+    x, y, limbs, dd = O.gendot(n, 1e16, 0)
+    ex = exact_value(limbs)
+    R = O.ref()
+    cores = os.cpu_count() or 1
+    # exact split of every product (Dekker's TwoProduct, no FMA needed)
+    f = 134217729.0
+    def split(v):
+        c = f * v
+        hi = c - (c - v)
+        return hi, v - hi
+    p = x * y
+    xh, xl = split(x)
+    yh, yl = split(y)
+    e = xl * yl - (((p - xh * yh) - xl * yh) - xh * yl)
+    terms = np.empty(2 * n)
+    terms[0::2] = p
+    terms[1::2] = e
+    if R is not None:
+        fns = {"dot2": lambda: R.ref_dot2(n, O.ptr(x), O.ptr(y)),
+               "dot_naive": lambda: R.ref_dot_naive(n, O.ptr(x), O.ptr(y)),
+               "dot2_threaded": lambda: R.ref_dot2_threaded(n, O.ptr(x), O.ptr(y), cores),
+               "sum2_of_products": lambda: R.ref_sum2_threaded(2 * n, O.ptr(terms), 1)}
+    else:
+        fns = {"dot2": lambda: O.dot2(x, y), "dot_naive": lambda: O.dot_naive(x, y),
+               "dot2_threaded": lambda: O.lib().orc_dot2_threaded(n, O.ptr(x), O.ptr(y), cores),
+               "sum2_of_products": lambda: O.sum2(terms)}
+    out = {"n": n, "cond_target": 1e16, "cond_achieved": float(dd[2]), "kind": "reference" if R else "port",
+           "threads": cores}
+    for k, fn in fns.items():
+        ts = []
+        for _ in range(7):
+            t0 = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        out[k] = {"rel_err": rel_err(r, ex), "ms": round(1e3 * sum(ts[1:-1]) / 5, 4)}
+    return out
+
+
+def cfg_cond_sweep(H, h, n, log):
+    """kappa 1e4..1e32 at N = 2^27 (seed 0): Dot2 GB/s, relative errors of Dot2 and naive dot."""
+    import numpy as np
+    L = H.lib()
+    x = H.DeviceArray(n, np.float64, h)
+    y = H.DeviceArray(n, np.float64, h)
+    d = H.DeviceArray(4, np.float64, h)
+    rows = []
+    for cond in (1e4, 1e8, 1e16, 1e24, 1e32):
+        log("cond sweep %g" % cond)
+        ex, dd = gendot(H, h, n, cond, 0, x, y)
+        ms = timed(h, L.hpb_dot2_f64_async, (h.ptr, n, x.ptr, y.ptr, d.ptr), 20, 3)
+        g = d.numpy()
+        out = np.zeros(4)
+        H.call("hpb_dot_f64", h.ptr, n, x.ptr, y.ptr, out.ctypes.data_as(ctypes.c_void_p))
+        rows.append({"cond_target": cond, "cond_achieved": float(dd[2]), "dot2_gbs": round(16.0 * n / ms / 1e6, 1),
+                     "dot2_rel_err": rel_err(g[2], ex), "naive_rel_err": rel_err(out[2], ex)})
+    for a in (x, y, d):
+        a.free()
+    return rows
+
+
+def cfg_blas1(H, h, x, y, n):
+    import numpy as np
+    L = H.lib()
+    z = H.DeviceArray(n, np.float64, h)
+    d = H.DeviceArray(4, np.float64, h)
+    H.call("hpb_copy_f64", h.ptr, n, y.ptr, z.ptr)
+    c = ctypes.c_double
+    res = {}
+    ms = timed(h, L.hpb_nrm2c_f64_async, (h.ptr, n, x.ptr, d.ptr), 20, 3)
+    res["nrm2c_gbs"] = round(8.0 * n / ms / 1e6, 1)
+    out = np.zeros(4)
+    ms = timed(h, L.hpb_sum2_f64, (h.ptr, n, x.ptr, out.ctypes.data_as(ctypes.c_void_p)), 20, 3)
+    res["sum2_gbs_with_readback"] = round(8.0 * n / ms / 1e6, 1)
+    ms = timed(h, L.hpb_axpy_f64, (h.ptr, n, c(1e-3), x.ptr, z.ptr), 20, 3)
+    res["axpy_gbs"] = round(24.0 * n / ms / 1e6, 1)
+    ms = timed(h, L.hpb_axpbyc_f64, (h.ptr, n, c(0.5), x.ptr, c(-0.25), y.ptr, z.ptr), 20, 3)
+    res["axpbyc_gbs"] = round(24.0 * n / ms / 1e6, 1)
+    ms = timed(h, L.hpb_dot_f64_async, (h.ptr, n, x.ptr, y.ptr, d.ptr), 20, 3)
+    res["dot_naive_gbs"] = round(16.0 * n / ms / 1e6, 1)
+    z.free()
+    d.free()
+    return res
+
+
+def cfg_spmv(H, h, log):
+    import numpy as np
+    import high_precision_blas1_amd.solvers as S
+    nx = 256
+    log("spmv 256^3")
+    A = S.DeviceCSR.stencil7(h, nx, nx, nx, S.POISSON7)
+    xs = H.DeviceArray(A.rows, np.float64, h)
+    ys = H.DeviceArray(A.rows, np.float64, h)
+    H.call("hpb_random_uniform_f64", h.ptr, A.rows, 1, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), xs.ptr)
+    nbytes = 12.0 * A.nnz + 4.0 * (A.rows + 1) + 16.0 * A.rows
+    L = H.lib()
+    res = {"rows": A.rows, "nnz": A.nnz, "algorithmic_bytes": int(nbytes)}
+    for name, fn in (("plain", L.hpb_csr_spmv_f64), ("compensated", L.hpb_csr_spmvc_f64)):
+        ms = timed(h, fn, (h.ptr, A.rows, A.rows, A.nnz, A.rp.ptr, A.col.ptr, A.val.ptr, ctypes.c_double(1.0), xs.ptr,
+                           ctypes.c_double(0.0), ys.ptr), 20, 3)
+        res[name + "_gbs"] = round(nbytes / ms / 1e6, 1)
+        res[name + "_ms"] = round(ms, 4)
+    A.free()
+    xs.free()
+    ys.free()
+    return res
+
+
+def _solve_problem(H, h, S, nx, coef, seed):
+    import numpy as np
+    A = S.DeviceCSR.stencil7(h, nx, nx, nx, coef)
+    xs = H.DeviceArray(A.rows, np.float64, h)
+    H.call("hpb_random_uniform_f64", h.ptr, A.rows, seed, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), xs.ptr)
+    b = H.DeviceArray(A.rows, np.float64, h)
+    A.spmv(xs, b, compensated=True)
+    xs.free()
+    return A, b
+
+
+def _run_solve(H, h, S, A, b, p, ir=False):
+    import numpy as np
+    x = H.DeviceArray(A.rows, np.float64, h)
+    H.call("hpb_fill_f64", h.ptr, A.rows, ctypes.c_double(0.0), x.ptr)
+    r = S.solve_ir(h, A, b, x, p, rel_tol=1e-12, max_outer=40) if ir else S.solve(h, A, b, x, p)
+    x.free()
+    return {k: (round(v, 6) if isinstance(v, float) and k == "seconds" else v)
+            for k, v in r.items() if k in ("converged", "iters", "outer_iters", "true_rel_resid", "seconds")}
+
+
+def cfg_krylov(H, h, log):
+    import high_precision_blas1_amd.solvers as S
+    log("krylov 128^3 convection-diffusion")
+    A, b = _solve_problem(H, h, S, 128, S.CONVDIFF7, 2)
+    res = {"matrix": "128^3 first-order upwind convection-diffusion, v=(1,0.5,0.25), cell Peclet 100",
+           "rows": A.rows, "nnz": A.nnz}
+    for name, p in (
+            ("gmres50_naive", S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000, high_precision=0)),
+            ("gmres50_compensated", S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000,
+                                             high_precision=1)),
+            ("gmres50_compensated_mgs_reference_schedule",
+             S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000, high_precision=1, gram_schmidt=0,
+                      early_exit=0)),
+            ("bicgstab4_naive", S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000,
+                                         high_precision=0)),
+            ("bicgstab4_compensated", S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000,
+                                               high_precision=1))):
+        log(name)
+        res[name] = _run_solve(H, h, S, A, b, p)
+    A.free()
+    b.free()
+    return res
+
+
+def cfg_mixed(H, h, log):
+    import high_precision_blas1_amd.solvers as S
+    log("mixed precision 128^3 convection-diffusion")
+    A, b = _solve_problem(H, h, S, 128, S.CONVDIFF7, 2)
+    res = {"matrix": "128^3 convection-diffusion (as krylov)"}
+    plain = {
+        "gmres_fp64": S.params(solver=S.GMRES, restart=30, rel_tol=1e-12, max_iters=20000, high_precision=1),
+        "bicgstab2_fp64": S.params(solver=S.BICGSTABL, basis=2, rel_tol=1e-12, max_iters=20000, high_precision=1)}
+    mixed = {
+        "gmres30_fp32_refined": S.params(solver=S.GMRES, restart=30, rel_tol=1e-5, max_iters=5000, high_precision=1),
+        "bicgstab2_fp32_refined": S.params(solver=S.BICGSTABL, basis=2, rel_tol=1e-5, max_iters=5000,
+                                           high_precision=1)}
+    for k, p in plain.items():
+        log(k)
+        res[k] = _run_solve(H, h, S, A, b, p)
+    for k, p in mixed.items():
+        log(k)
+        res[k] = _run_solve(H, h, S, A, b, p, ir=True)
+    for a, m in (("gmres_fp64", "gmres30_fp32_refined"), ("bicgstab2_fp64", "bicgstab2_fp32_refined")):
+        if res[a]["converged"] and res[m]["converged"]:
+            res["speedup_" + m] = round(res[a]["seconds"] / res[m]["seconds"], 3)
+    A.free()
+    b.free()
+    return res
+
+
+def cfg_multidot(H, h, log):
+    """Fused compensated multi-dot, k = 1..64 at n = 2^24 (w read once per 8 columns)."""
+    import numpy as np
+    log("multidot")
+    n = 1 << 24
+    kmax = 64
+    V = [H.DeviceArray(n, np.float64, h) for _ in range(kmax)]
+    w = H.DeviceArray(n, np.float64, h)
+    for i, a in enumerate(V + [w]):
+        H.call("hpb_random_uniform_f64", h.ptr, n, 5, i, ctypes.c_double(-1.0), ctypes.c_double(1.0), a.ptr)
+    d = H.DeviceArray(4 * kmax, np.float64, h)
+    L = H.lib()
+    rows = []
+    for k in (1, 2, 4, 8, 16, 32, 50, 64):
+        ptrs = (ctypes.c_void_p * k)(*[a.addr for a in V[:k]])
+        ms = timed(h, L.hpb_multidot_f64_async, (h.ptr, k, ptrs, n, w.ptr, 1, d.ptr), 10, 2)
+        nbytes = 8.0 * n * (k + 1)
+        rows.append({"k": k, "ms": round(ms, 4), "gbs_algorithmic": round(nbytes / ms / 1e6, 1),
+                     "separate_dot2_bytes_ratio": round(2.0 * k / (k + 1), 3)})
+    for a in V + [w, d]:
+        a.free()
+    return {"n": n, "rows": rows}
+
+
+def run_suite(H, h, x, y, n, args, log, no_cpu=False):
+    res = {}
+    if not no_cpu:
+        log("cpu plain config")
+        res["cpu_plain"] = cfg_cpu_plain()
+    res["blas1"] = cfg_blas1(H, h, x, y, n)
+    res["cond_sweep"] = cfg_cond_sweep(H, h, n, log)
+    res["spmv"] = cfg_spmv(H, h, log)
+    res["krylov"] = cfg_krylov(H, h, log)
+    res["mixed"] = cfg_mixed(H, h, log)
+    res["multidot"] = cfg_multidot(H, h, log)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,23 +361,28 @@ def main():
     ap.add_argument("--cond", type=float, default=1e16)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the hot path is measured on one MI355X; "
               "--gpus %d is not supported)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import high_precision_blas1_amd as H
     h = H.Handle()
+    L = H.lib()
     n = args.n
-    log("bench: generating inputs n=%d" % n)
-    x, y, gen = make_inputs(H, h, n, args.cond, args.seed)
-    h.sync()
-    log("bench: timing %d steps after %d warmup" % (args.steps, args.warmup))
-    ms, res = time_dot2(H, h, x, y, n, args.steps, args.warmup)
-    per = ms / args.steps
+    log("generating GenDot pair n=%d kappa=%g" % (n, args.cond))
+    x = H.DeviceArray(n, np.float64, h)
+    y = H.DeviceArray(n, np.float64, h)
+    exact, dd = gendot(H, h, n, args.cond, args.seed, x, y)
+    dout = H.DeviceArray(4, np.float64, h)
+    log("timing %d Dot2 steps after %d warmup" % (args.steps, args.warmup))
+    ms = timed(h, L.hpb_dot2_f64_async, (h.ptr, n, x.ptr, y.ptr, dout.ptr), args.steps, args.warmup)
+    res = dout.numpy()
     nbytes = 16.0 * n
-    gbs = nbytes / (per * 1e-3) / 1e9
+    gbs = nbytes / (ms * 1e-3) / 1e9
     cfg = h.reduce_config("float64", n, True)
     out = {
         "metric": METRIC,
@@ -147,27 +391,31 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(per, 5),
+        "ms_per_step": round(ms, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
+        "data": "synthetic (device GenDot pair, kappa %g, seed %d)" % (args.cond, args.seed),
         "config": {"workload": "FP64 compensated Dot2 (Ogita-Rump-Oishi), N=2^%d per vector, device-resident"
-                               % (n.bit_length() - 1), "n": n, "input": gen,
+                               % (n.bit_length() - 1), "n": n,
                    "launch": {"block": cfg[0], "unroll": cfg[1], "grid": cfg[2], "vector_width": cfg[3]}},
         "hbm_peak_frac": round(gbs / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_reduce<double,DOT2> (hpb_dot2_f64)",
+                     "kernel": "hpb::k_reduce<double, DOT2> (hpb_dot2_f64)",
                      "algorithmic_bytes_per_launch": int(nbytes)},
-        "result": {"s": float(res[0]), "c": float(res[1]), "dot": float(res[2])},
+        "result": {"dot2": float(res[2]), "exact": float(exact), "rel_err": rel_err(res[2], exact),
+                   "cond_achieved": float(dd[2])},
     }
     if not args.no_cpu:
-        log("bench: CPU baseline")
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(H, h, x, y, n)
+    if not args.headline_only:
+        out["configs"] = run_suite(H, h, x, y, n, args, log, no_cpu=args.no_cpu)
     x.free()
     y.free()
+    dout.free()
     h.close()
     print(json.dumps(out), flush=True)
 

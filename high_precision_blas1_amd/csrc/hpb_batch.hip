@@ -1,0 +1,634 @@
+// hpb_batch.hip -- many independent CSR systems solved together by BiCGStab(2),
+// one captured hipGraph per BiCGStab(2) cycle.
+//
+// No reference counterpart: the reference solves one system at a time with a host
+// round trip per dot (numerical_algos/lin_solvers/bicgstabl.h:151-274).  Here all
+// systems live in one block-diagonal CSR matrix; rows are cut into tiles that never
+// cross a system, so every kernel finds its system's scalars from the tile.
+//   * segmented compensated dots: per-tile (s, c) partials, merged per system in
+//     tile order by one wave (reproducible);
+//   * per-system scalar recurrences on the device (one lane per system);
+//   * updates and SpMV over all systems in one launch each; tiles of converged
+//     systems return at once;
+//   * the whole cycle (5 SpMV incl. the true-residual recomputation, 6 dot groups,
+//     5 update kernels, 6 scalar kernels) is captured once as a hipGraph and
+//     replayed; the host reads the count of active systems every few cycles.
+// The recurrences are the reference's BiCGStab(L) for L = 2 (bicgstabl.h:199-258)
+// with the MR part evaluated from one fused Gram matrix pass: tau12 = (r2,r1)/s1,
+// s2 = (r2,r2) - 2 tau12 (r2,r1) + tau12^2 s1, which equals the modified
+// Gram-Schmidt values in exact arithmetic (the dots are compensated).
+//
+// Test matrices: 2-D 5-point diffusion, cell coefficients exp(sigma g), g ~ N(0,1)
+// from Philox (key seed + s * 0x9E3779B97F4A7C15, counter = cell), harmonic face
+// means as in hpb_csr_diffusion2d_f64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <type_traits>
+#include <cmath>
+#include <vector>
+
+#include "hpb_device.hpp"
+#include "hpb_internal.hpp"
+#include "hpb_philox.hpp"
+
+namespace hpb {
+
+constexpr int BT_ROWS = 2048;  // rows per tile
+constexpr int BT_THREADS = 256;
+constexpr int BQ = 5;          // dot products per group, at most
+
+struct btile {
+    int sys;
+    int pad;
+    int64_t r0, r1;  // global rows [r0, r1)
+};
+
+// state per system
+enum { S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BNORM, S_RNORM, S_G1, S_GPP1, S_GP1, S_GP2, S_TAU, S_G2, S_NSTATE };
+
+struct hpb_batch_s;
+struct batch_t {
+    hpb_handle_t h = nullptr;
+    int nsys = 0;
+    int64_t nrows = 0, nnz = 0;
+    int ntiles = 0;
+    std::vector<int64_t> row_off;   // nsys + 1
+    std::vector<int> tile_first;    // nsys + 1
+    btile* d_tiles = nullptr;
+    int* d_tile_first = nullptr;
+    int64_t* d_row_off = nullptr;
+    int* rp = nullptr;
+    int* col = nullptr;
+    double* val = nullptr;
+    double* v[10] = {};             // b, x, r0, r1, r2, u0, u1, u2, rt, xs
+    double* st = nullptr;           // nsys * S_NSTATE
+    int* active = nullptr;          // nsys
+    int* iters = nullptr;           // nsys
+    int* nactive = nullptr;         // 1
+    double* part = nullptr;         // ntiles * BQ * 2
+    double* dots = nullptr;         // nsys * BQ
+    int* hcount = nullptr;          // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+enum { VB = 0, VX, VR0, VR1, VR2, VU0, VU1, VU2, VRT, VXS };
+
+// ---------------------------------------------------------------- builders
+
+__device__ __forceinline__ double cell_kappa(int m, int i, int j, double sigma, uint64_t seed)
+{
+    const int64_t c = (int64_t)i + (int64_t)m * j;
+    philox_out o = philox4x32_10((uint32_t)c, (uint32_t)(c >> 32), 0u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = unif53_open0(k53(o.v[0], o.v[1]));
+    const double u2 = unif53(k53(o.v[2], o.v[3]));
+    return exp(sigma * (sqrt(-2.0 * log(u1)) * cospi(2.0 * u2)));
+}
+
+// entries of the 5-point stencil rows before row (i, j) of an m x m grid
+__device__ __forceinline__ int64_t rows_before(int m, int i, int j)
+{
+    const int64_t full = (int64_t)j * (3 * m - 2) + (int64_t)m * ((j > 0 ? j - 1 : 0) + (j < m - 1 ? j : m - 1));
+    const int ny = (j > 0) + (j < m - 1);
+    const int64_t part = (int64_t)i + (i > 0 ? i - 1 : 0) + (i < m - 1 ? i : m - 1) + (int64_t)i * ny;
+    return full + part;
+}
+
+__device__ __forceinline__ int find_sys(const int64_t* row_off, int nsys, int64_t r)
+{
+    int lo = 0, hi = nsys;  // row_off[lo] <= r < row_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (row_off[mid] <= r) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_build(int nsys, int64_t nrows, const int64_t* __restrict__ row_off,
+                                                     const int64_t* __restrict__ nnz_off, const int* __restrict__ side,
+                                                     double sigma, uint64_t seed, int* __restrict__ rp,
+                                                     int* __restrict__ col, double* __restrict__ val)
+{
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * 256) {
+        const int s = find_sys(row_off, nsys, r);
+        const int m = side[s];
+        const int64_t lr = r - row_off[s];
+        const int i = (int)(lr % m), j = (int)(lr / m);
+        const uint64_t sd = seed + (uint64_t)s * 0x9E3779B97F4A7C15ull;
+        const double kc = cell_kappa(m, i, j, sigma, sd);
+        auto face = [&](int ii, int jj) {
+            const double kn = cell_kappa(m, ii, jj, sigma, sd);
+            return 2.0 * kc * kn / (kc + kn);
+        };
+        const double fs = j > 0 ? face(i, j - 1) : kc;
+        const double fw = i > 0 ? face(i - 1, j) : kc;
+        const double fe = i < m - 1 ? face(i + 1, j) : kc;
+        const double fn = j < m - 1 ? face(i, j + 1) : kc;
+        const int64_t p0 = nnz_off[s] + rows_before(m, i, j);
+        int64_t p = p0;
+        const int64_t g = row_off[s];
+        if (j > 0) { col[p] = (int)(g + lr - m); val[p++] = -fs; }
+        if (i > 0) { col[p] = (int)(g + lr - 1); val[p++] = -fw; }
+        col[p] = (int)r; val[p++] = ((fs + fw) + fe) + fn;
+        if (i < m - 1) { col[p] = (int)(g + lr + 1); val[p++] = -fe; }
+        if (j < m - 1) { col[p] = (int)(g + lr + m); val[p++] = -fn; }
+        rp[r] = (int)p0;
+        if (r == nrows - 1) rp[nrows] = (int)p;
+    }
+}
+
+// ---------------------------------------------------------------- kernels over tiles
+
+// y = A x (MODE 0) or y = b - A x (MODE 1, compensated rows) for active systems
+template <int MODE>
+__global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ tiles, const int* __restrict__ active,
+                                                      const int* __restrict__ rp, const int* __restrict__ col,
+                                                      const double* __restrict__ val, const double* __restrict__ x,
+                                                      const double* __restrict__ b, double* __restrict__ y)
+{
+    const btile t = tiles[blockIdx.x];
+    if (!active[t.sys]) return;
+    constexpr int LPR = 4;
+    const int lane = threadIdx.x % LPR;
+    for (int64_t r = t.r0 + threadIdx.x / LPR; r < t.r1; r += BT_THREADS / LPR) {
+        double s = (MODE == 1 && lane == 0) ? b[r] : 0.0, c = 0.0;
+        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
+            const double a = MODE == 1 ? -val[k] : val[k];
+            dev::dot2_step<double>(s, c, a, x[col[k]]);
+        }
+#pragma unroll
+        for (int m = LPR / 2; m >= 1; m >>= 1) {
+            double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
+            dev::merge<double>(s, c, s2, c2);
+        }
+        if (lane == 0) y[r] = s + c;
+    }
+}
+
+struct dot_pairs {
+    const double* a[BQ];
+    const double* b[BQ];
+};
+
+// per-tile compensated partials of NQ dot products
+template <int NQ>
+__global__ __launch_bounds__(BT_THREADS) void k_bdots(const btile* __restrict__ tiles, const int* __restrict__ active,
+                                                      dot_pairs P, double* __restrict__ part)
+{
+    const btile t = tiles[blockIdx.x];
+    if (!active[t.sys]) return;
+    double s[NQ], c[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) s[q] = c[q] = 0.0;
+    for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += BT_THREADS) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) dev::dot2_step<double>(s[q], c[q], P.a[q][r], P.b[q][r]);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        dev::block_reduce<double, true, BT_THREADS>(s[q], c[q]);
+        if (threadIdx.x == 0) {
+            part[((size_t)blockIdx.x * BQ + q) * 2] = s[q];
+            part[((size_t)blockIdx.x * BQ + q) * 2 + 1] = c[q];
+        }
+    }
+}
+
+// one wave per system: merge the system's tile partials in tile order
+template <int NQ>
+__device__ __forceinline__ void merge_sys(int s, const int* tile_first, const double* part, double* d)
+{
+    const int lane = threadIdx.x & 63;
+    for (int q = 0; q < NQ; ++q) {
+        double fs = 0.0, fc = 0.0;
+        for (int t = tile_first[s] + lane; t < tile_first[s + 1]; t += 64)
+            dev::merge<double>(fs, fc, part[((size_t)t * BQ + q) * 2], part[((size_t)t * BQ + q) * 2 + 1]);
+        dev::wave_reduce<double, true>(fs, fc);
+        d[q] = fs + fc;
+    }
+}
+
+// phase: 0 rho1 -> beta; 1 alpha; 2 MR; 3 residual check; 4 init (rnorm, bnorm)
+template <int NQ, int PHASE>
+__global__ __launch_bounds__(256) void k_bsys(int nsys, const int* __restrict__ tile_first,
+                                              const double* __restrict__ part, double* __restrict__ st,
+                                              int* __restrict__ active, int* __restrict__ iters, int* __restrict__ nactive,
+                                              double tol2, int first)
+{
+    const int s = (blockIdx.x * 256 + threadIdx.x) >> 6;
+    if (s >= nsys || !active[s]) return;
+    double d[NQ];
+    merge_sys<NQ>(s, tile_first, part, d);
+    if ((threadIdx.x & 63) != 0) return;
+    double* S = st + (size_t)s * S_NSTATE;
+    if (PHASE == 0) {  // d0 = (r_j, rt)
+        double rho = S[S_RHO];
+        if (first) rho = -S[S_OMEGA] * rho;
+        if (rho == 0.0) {
+            active[s] = 0;
+            S[S_BETA] = 0.0;
+            return;
+        }
+        const double rho1 = d[0];
+        S[S_BETA] = S[S_ALPHA] * rho1 / rho;
+        S[S_RHO] = rho1;
+    } else if (PHASE == 1) {  // d0 = (u_{j+1}, rt)
+        S[S_ALPHA] = S[S_RHO] / d[0];
+    } else if (PHASE == 2) {  // d = (r1,r1), (r2,r1), (r2,r2), (r0,r1), (r0,r2)
+        const double s1 = d[0];
+        const double tau = d[1] / s1;
+        const double s2 = d[2] - 2.0 * tau * d[1] + tau * tau * s1;
+        const double gp1 = d[3] / s1;
+        const double gp2 = (d[4] - tau * d[3]) / s2;
+        const double g2 = gp2;
+        const double g1 = gp1 - tau * g2;
+        S[S_TAU] = tau;
+        S[S_GP1] = gp1;
+        S[S_GP2] = gp2;
+        S[S_G2] = g2;
+        S[S_G1] = g1;
+        S[S_GPP1] = g2;
+        S[S_OMEGA] = g2;
+        iters[s] += 2;
+    } else if (PHASE == 3) {  // d0 = (r0, r0) of the true residual
+        S[S_RNORM] = d[0];
+        if (!(d[0] <= tol2 * S[S_BNORM])) {
+            if (d[0] == d[0]) atomicAdd(nactive, 1);
+            else active[s] = 0;
+        } else {
+            active[s] = 0;
+        }
+    } else {  // init: d0 = (b, b), d1 = (r0, r0)
+        S[S_BNORM] = d[0];
+        S[S_RNORM] = d[1];
+        S[S_RHO] = 1.0;
+        S[S_ALPHA] = 0.0;
+        S[S_OMEGA] = 1.0;
+        if (d[1] <= tol2 * d[0]) active[s] = 0;
+    }
+}
+
+// updates; every tile reads its system's scalars
+template <int KIND>
+__global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ tiles, const int* __restrict__ active,
+                                                     const double* __restrict__ st, double* __restrict__ x,
+                                                     double* __restrict__ r0, double* __restrict__ r1,
+                                                     double* __restrict__ r2, double* __restrict__ u0,
+                                                     double* __restrict__ u1, double* __restrict__ u2)
+{
+    const btile t = tiles[blockIdx.x];
+    if (!active[t.sys]) return;
+    const double* S = st + (size_t)t.sys * S_NSTATE;
+    const double beta = S[S_BETA], alpha = S[S_ALPHA];
+    for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += BT_THREADS) {
+        if (KIND == 0) {  // u0 = r0 - beta u0
+            u0[r] = dev::fma_rn<double>(-beta, u0[r], r0[r]);
+        } else if (KIND == 1) {  // r0 -= alpha u1 ; x += alpha u0
+            r0[r] = dev::fma_rn<double>(-alpha, u1[r], r0[r]);
+            x[r] = dev::fma_rn<double>(alpha, u0[r], x[r]);
+        } else if (KIND == 2) {  // u0 = r0 - beta u0 ; u1 = r1 - beta u1
+            u0[r] = dev::fma_rn<double>(-beta, u0[r], r0[r]);
+            u1[r] = dev::fma_rn<double>(-beta, u1[r], r1[r]);
+        } else if (KIND == 3) {  // r0 -= alpha u1 ; r1 -= alpha u2 ; x += alpha u0
+            r0[r] = dev::fma_rn<double>(-alpha, u1[r], r0[r]);
+            r1[r] = dev::fma_rn<double>(-alpha, u2[r], r1[r]);
+            x[r] = dev::fma_rn<double>(alpha, u0[r], x[r]);
+        } else {  // MR update
+            const double g1 = S[S_G1], gpp1 = S[S_GPP1], gp1 = S[S_GP1], gp2 = S[S_GP2], tau = S[S_TAU],
+                         g2 = S[S_G2];
+            const double a0 = r0[r], a1 = r1[r], a2 = r2[r];
+            x[r] = dev::fma_rn<double>(gpp1, a1, dev::fma_rn<double>(g1, a0, x[r]));
+            r0[r] = dev::fma_rn<double>(gp2 * tau - gp1, a1, dev::fma_rn<double>(-gp2, a2, a0));
+            u0[r] = dev::fma_rn<double>(-g1, u1[r], dev::fma_rn<double>(-g2, u2[r], u0[r]));
+        }
+    }
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+// ---------------------------------------------------------------- host side
+
+static int free_batch(batch_t* B)
+{
+    if (!B) return HPB_OK;
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    void* ptrs[] = {B->d_tiles, B->d_tile_first, B->d_row_off, B->rp, B->col, B->val, B->st, B->active,
+                    B->iters, B->nactive, B->part, B->dots};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (double* p : B->v)
+        if (p) (void)hipFree(p);
+    if (B->hcount) (void)hipHostFree(B->hcount);
+    delete B;
+    return HPB_OK;
+}
+
+static unsigned sys_blocks(int nsys) { return (unsigned)((nsys * 64 + 255) / 256); }
+
+// record one BiCGStab(2) cycle on the batch's stream
+static void record_cycle(batch_t* B, double tol2)
+{
+    hipStream_t st = B->h->stream;
+    const unsigned T = (unsigned)B->ntiles, SB = sys_blocks(B->nsys);
+    double** v = B->v;
+    auto dots1 = [&](const double* a, const double* b) {
+        dot_pairs P{};
+        P.a[0] = a;
+        P.b[0] = b;
+        hipLaunchKernelGGL((k_bdots<1>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+    };
+    auto spmv = [&](const double* x, double* y) {
+        hipLaunchKernelGGL((k_bspmv<0>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, B->rp, B->col,
+                           B->val, x, (const double*)nullptr, y);
+    };
+    auto upd = [&](auto kind) {
+        hipLaunchKernelGGL((k_bupd<decltype(kind)::value>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active,
+                           B->st, v[VX], v[VR0], v[VR1], v[VR2], v[VU0], v[VU1], v[VU2]);
+    };
+    auto sys = [&](auto nq, auto phase, int first) {
+        hipLaunchKernelGGL((k_bsys<decltype(nq)::value, decltype(phase)::value>), dim3(SB), dim3(256), 0, st, B->nsys,
+                           B->d_tile_first, B->part, B->st, B->active, B->iters, B->nactive, tol2, first);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, B->nactive);
+    // j = 0
+    dots1(v[VR0], v[VRT]);
+    sys(I1{}, I0{}, 1);
+    upd(I0{});                       // u0 = r0 - beta u0
+    spmv(v[VU0], v[VU1]);            // u1 = A u0
+    dots1(v[VU1], v[VRT]);
+    sys(I1{}, I1{}, 0);              // alpha
+    upd(I1{});                       // r0 -= alpha u1, x += alpha u0
+    spmv(v[VR0], v[VR1]);            // r1 = A r0
+    // j = 1
+    dots1(v[VR1], v[VRT]);
+    sys(I1{}, I0{}, 0);
+    upd(I2{});                       // u0, u1
+    spmv(v[VU1], v[VU2]);            // u2 = A u1
+    dots1(v[VU2], v[VRT]);
+    sys(I1{}, I1{}, 0);
+    upd(I3{});                       // r0, r1, x
+    spmv(v[VR1], v[VR2]);            // r2 = A r1
+    // MR part
+    {
+        dot_pairs P{};
+        P.a[0] = v[VR1]; P.b[0] = v[VR1];
+        P.a[1] = v[VR2]; P.b[1] = v[VR1];
+        P.a[2] = v[VR2]; P.b[2] = v[VR2];
+        P.a[3] = v[VR0]; P.b[3] = v[VR1];
+        P.a[4] = v[VR0]; P.b[4] = v[VR2];
+        hipLaunchKernelGGL((k_bdots<5>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+    }
+    sys(I5{}, I2{}, 0);
+    upd(I4{});
+    // true residual r0 = b - A x and the convergence check
+    hipLaunchKernelGGL((k_bspmv<1>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, B->rp, B->col, B->val,
+                       v[VX], v[VB], v[VR0]);
+    dots1(v[VR0], v[VR0]);
+    sys(I1{}, I3{}, 0);
+}
+
+}  // namespace hpb
+
+using namespace hpb;
+
+extern "C" {
+
+int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, double sigma, uint64_t seed,
+                                 uint64_t rhs_seed, hpb_batch_t* out)
+{
+    HPB_CLEAR_STALE();
+    HPB_REQUIRE(h && sides && out && nsys >= 1, HPB_ERR_INVALID, "hpb_batch_create_diffusion2d: bad argument");
+    *out = nullptr;
+    batch_t* B = new batch_t();
+    B->h = h;
+    B->nsys = nsys;
+    B->row_off.assign(nsys + 1, 0);
+    std::vector<int64_t> nnz_off(nsys + 1, 0);
+    std::vector<btile> tiles;
+    B->tile_first.assign(nsys + 1, 0);
+    for (int s = 0; s < nsys; ++s) {
+        const int64_t m = sides[s];
+        if (m < 2 || m > 46340) {
+            free_batch(B);
+            HPB_REQUIRE(false, HPB_ERR_INVALID, "hpb_batch_create_diffusion2d: side length must be in [2, 46340]");
+        }
+        B->row_off[s + 1] = B->row_off[s] + m * m;
+        nnz_off[s + 1] = nnz_off[s] + m * m + 4 * m * (m - 1);
+        B->tile_first[s] = (int)tiles.size();
+        for (int64_t r = B->row_off[s]; r < B->row_off[s + 1]; r += BT_ROWS)
+            tiles.push_back(btile{s, 0, r, std::min<int64_t>(r + BT_ROWS, B->row_off[s + 1])});
+    }
+    B->tile_first[nsys] = (int)tiles.size();
+    B->nrows = B->row_off[nsys];
+    B->nnz = nnz_off[nsys];
+    B->ntiles = (int)tiles.size();
+    if (B->nnz >= (1ll << 31)) {
+        free_batch(B);
+        HPB_REQUIRE(false, HPB_ERR_INVALID, "hpb_batch_create_diffusion2d: more than 2^31-1 non-zeros");
+    }
+    hipError_t e = hipSuccess;
+    auto M = [&](void** p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes);
+    };
+    M((void**)&B->d_tiles, sizeof(btile) * B->ntiles);
+    M((void**)&B->d_tile_first, sizeof(int) * (nsys + 1));
+    M((void**)&B->d_row_off, sizeof(int64_t) * (nsys + 1));
+    M((void**)&B->rp, sizeof(int) * (B->nrows + 1));
+    M((void**)&B->col, sizeof(int) * B->nnz);
+    M((void**)&B->val, sizeof(double) * B->nnz);
+    for (int k = 0; k < 10; ++k) M((void**)&B->v[k], sizeof(double) * B->nrows);
+    M((void**)&B->st, sizeof(double) * S_NSTATE * nsys);
+    M((void**)&B->active, sizeof(int) * nsys);
+    M((void**)&B->iters, sizeof(int) * nsys);
+    M((void**)&B->nactive, sizeof(int));
+    M((void**)&B->part, sizeof(double) * 2 * BQ * B->ntiles);
+    M((void**)&B->dots, sizeof(double) * BQ * nsys);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&B->hcount, sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_batch(B);
+        return hpb::hip_status(e, __FILE__, __LINE__, "hpb_batch_create_diffusion2d: allocation");
+    }
+    int64_t* d_nnz_off = nullptr;
+    int* d_side = nullptr;
+    HPB_HIP(hipMalloc((void**)&d_nnz_off, sizeof(int64_t) * (nsys + 1)));
+    HPB_HIP(hipMalloc((void**)&d_side, sizeof(int) * nsys));
+    hipStream_t st = h->stream;
+    HPB_HIP(hipMemcpyAsync(B->d_tiles, tiles.data(), sizeof(btile) * B->ntiles, hipMemcpyHostToDevice, st));
+    HPB_HIP(hipMemcpyAsync(B->d_tile_first, B->tile_first.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, st));
+    HPB_HIP(hipMemcpyAsync(B->d_row_off, B->row_off.data(), sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice, st));
+    HPB_HIP(hipMemcpyAsync(d_nnz_off, nnz_off.data(), sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice, st));
+    HPB_HIP(hipMemcpyAsync(d_side, sides, sizeof(int) * nsys, hipMemcpyHostToDevice, st));
+    int64_t g = (B->nrows + 255) / 256;
+    const int64_t cap = (int64_t)(h->num_cu > 0 ? h->num_cu : 256) * 16;
+    if (g > cap) g = cap;
+    hipLaunchKernelGGL(k_batch_build, dim3((unsigned)g), dim3(256), 0, st, nsys, B->nrows, B->d_row_off, d_nnz_off,
+                       d_side, sigma, seed, B->rp, B->col, B->val);
+    HPB_LAUNCH_CHECK();
+    // x* uniform in [-1, 1] (Philox, rhs_seed), b = A x*
+    HPB_HIP(hipStreamSynchronize(st));
+    (void)hipFree(d_nnz_off);
+    (void)hipFree(d_side);
+    int rc = hpb_random_uniform_f64(h, (size_t)B->nrows, rhs_seed, 0, -1.0, 1.0, B->v[VXS]);
+    if (rc) {
+        free_batch(B);
+        return rc;
+    }
+    rc = hpb_csr_spmvc_f64(h, (int)B->nrows, (int)B->nrows, B->nnz, B->rp, B->col, B->val, 1.0, B->v[VXS], 0.0,
+                           B->v[VB]);
+    if (rc) {
+        free_batch(B);
+        return rc;
+    }
+    HPB_HIP(hipStreamSynchronize(st));
+    *out = (hpb_batch_t)B;
+    return HPB_OK;
+}
+
+int hpb_batch_info(hpb_batch_t b, int64_t* rows, int64_t* nnz, int* ntiles)
+{
+    HPB_REQUIRE(b, HPB_ERR_INVALID, "hpb_batch_info: null batch");
+    batch_t* B = (batch_t*)b;
+    if (rows) *rows = B->nrows;
+    if (nnz) *nnz = B->nnz;
+    if (ntiles) *ntiles = B->ntiles;
+    return HPB_OK;
+}
+
+int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int use_graph, int check_every,
+                              int* out_iters, double* out_true_rel_resid, int* out_converged, double* out_seconds,
+                              int* out_cycles)
+{
+    HPB_CLEAR_STALE();
+    HPB_REQUIRE(b && rel_tol > 0 && max_iters >= 2 && check_every >= 1, HPB_ERR_INVALID,
+                "hpb_batch_solve_bicgstab2: bad argument");
+    batch_t* B = (batch_t*)b;
+    hpb_handle_t h = B->h;
+    hipStream_t st = h->stream;
+    const double tol2 = rel_tol * rel_tol;
+    double** v = B->v;
+    const size_t nb = sizeof(double) * B->nrows;
+    // x = 0, r0 = b, rt = r0 (unnormalised: scaling r~ rescales rho and (u, r~) alike), u0 = 0
+    HPB_HIP(hipMemsetAsync(v[VX], 0, nb, st));
+    HPB_HIP(hipMemsetAsync(v[VU0], 0, nb, st));
+    HPB_HIP(hipMemsetAsync(v[VU1], 0, nb, st));
+    HPB_HIP(hipMemsetAsync(v[VU2], 0, nb, st));
+    HPB_HIP(hipMemcpyAsync(v[VR0], v[VB], nb, hipMemcpyDeviceToDevice, st));
+    HPB_HIP(hipMemcpyAsync(v[VRT], v[VB], nb, hipMemcpyDeviceToDevice, st));
+    HPB_HIP(hipMemsetAsync(B->iters, 0, sizeof(int) * B->nsys, st));
+    {
+        std::vector<int> ones(B->nsys, 1);
+        HPB_HIP(hipMemcpyAsync(B->active, ones.data(), sizeof(int) * B->nsys, hipMemcpyHostToDevice, st));
+        HPB_HIP(hipStreamSynchronize(st));
+    }
+    {
+        dot_pairs P{};
+        P.a[0] = v[VB];
+        P.b[0] = v[VB];
+        P.a[1] = v[VR0];
+        P.b[1] = v[VR0];
+        hipLaunchKernelGGL((k_bdots<2>), dim3(B->ntiles), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+        hipLaunchKernelGGL((k_bsys<2, 4>), dim3(sys_blocks(B->nsys)), dim3(256), 0, st, B->nsys, B->d_tile_first,
+                           B->part, B->st, B->active, B->iters, B->nactive, tol2, 0);
+        HPB_LAUNCH_CHECK();
+    }
+    if (use_graph && !B->exec) {
+        HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        record_cycle(B, tol2);
+        hipError_t ce = hipStreamEndCapture(st, &B->graph);
+        if (ce != hipSuccess) return hpb::hip_status(ce, __FILE__, __LINE__, "hipStreamEndCapture");
+        HPB_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    }
+    HPB_HIP(hipStreamSynchronize(st));
+    auto t0 = std::chrono::steady_clock::now();
+    int cycles = 0;
+    const int max_cycles = max_iters / 2;
+    while (cycles < max_cycles) {
+        const int k = std::min(check_every, max_cycles - cycles);
+        for (int i = 0; i < k; ++i) {
+            if (use_graph) HPB_HIP(hipGraphLaunch(B->exec, st));
+            else record_cycle(B, tol2);
+        }
+        HPB_LAUNCH_CHECK();
+        cycles += k;
+        HPB_HIP(hipMemcpyAsync(B->hcount, B->nactive, sizeof(int), hipMemcpyDeviceToHost, st));
+        HPB_HIP(hipStreamSynchronize(st));
+        if (*B->hcount == 0) break;
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // per-system true relative residuals, compensated
+    std::vector<int> ones(B->nsys, 1);
+    HPB_HIP(hipMemcpyAsync(B->active, ones.data(), sizeof(int) * B->nsys, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL((k_bspmv<1>), dim3(B->ntiles), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, B->rp, B->col,
+                       B->val, v[VX], v[VB], v[VR1]);
+    {
+        dot_pairs P{};
+        P.a[0] = v[VR1];
+        P.b[0] = v[VR1];
+        P.a[1] = v[VB];
+        P.b[1] = v[VB];
+        hipLaunchKernelGGL((k_bdots<2>), dim3(B->ntiles), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+    }
+    HPB_LAUNCH_CHECK();
+    std::vector<double> part((size_t)B->ntiles * BQ * 2);
+    std::vector<int> it(B->nsys);
+    HPB_HIP(hipMemcpyAsync(part.data(), B->part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    HPB_HIP(hipMemcpyAsync(it.data(), B->iters, sizeof(int) * B->nsys, hipMemcpyDeviceToHost, st));
+    HPB_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < B->nsys; ++s) {
+        double rr = 0.0, bb = 0.0;
+        for (int t = B->tile_first[s]; t < B->tile_first[s + 1]; ++t) {
+            rr += part[((size_t)t * BQ + 0) * 2] + part[((size_t)t * BQ + 0) * 2 + 1];
+            bb += part[((size_t)t * BQ + 1) * 2] + part[((size_t)t * BQ + 1) * 2 + 1];
+        }
+        const double rel = std::sqrt(rr) / std::sqrt(bb);
+        if (out_true_rel_resid) out_true_rel_resid[s] = rel;
+        if (out_iters) out_iters[s] = it[s];
+        if (out_converged) out_converged[s] = rel <= rel_tol ? 1 : 0;
+    }
+    if (out_seconds) *out_seconds = secs;
+    if (out_cycles) *out_cycles = cycles;
+    return HPB_OK;
+}
+
+int hpb_batch_solution(hpb_batch_t b, double* host_x, double* host_xstar)
+{
+    HPB_REQUIRE(b, HPB_ERR_INVALID, "hpb_batch_solution: null batch");
+    batch_t* B = (batch_t*)b;
+    hipStream_t st = B->h->stream;
+    if (host_x) HPB_HIP(hipMemcpyAsync(host_x, B->v[VX], sizeof(double) * B->nrows, hipMemcpyDeviceToHost, st));
+    if (host_xstar) HPB_HIP(hipMemcpyAsync(host_xstar, B->v[VXS], sizeof(double) * B->nrows, hipMemcpyDeviceToHost, st));
+    HPB_HIP(hipStreamSynchronize(st));
+    return HPB_OK;
+}
+
+int hpb_batch_csr(hpb_batch_t b, const int** row_ptr, const int** col_ind, const double** val, const double** rhs)
+{
+    HPB_REQUIRE(b, HPB_ERR_INVALID, "hpb_batch_csr: null batch");
+    batch_t* B = (batch_t*)b;
+    if (row_ptr) *row_ptr = B->rp;
+    if (col_ind) *col_ind = B->col;
+    if (val) *val = B->val;
+    if (rhs) *rhs = B->v[VB];
+    return HPB_OK;
+}
+
+int hpb_batch_destroy(hpb_batch_t b)
+{
+    if (b) {
+        batch_t* B = (batch_t*)b;
+        if (B->h) (void)hipStreamSynchronize(B->h->stream);
+        free_batch(B);
+    }
+    return HPB_OK;
+}
+
+}  // extern "C"

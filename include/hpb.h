@@ -326,6 +326,31 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
 int hpb_scale_convert_f64_to_f32(hpb_handle_t h, size_t n, double s, const double* x, float* y);
 int hpb_axpy_f32_to_f64(hpb_handle_t h, size_t n, double a, const float* x, double* y);
 
+/* ---- batched solves (new: the reference solves one system at a time) -------- */
+/* Many independent 2-D log-normal diffusion systems (side m_s, cells exp(sigma g),
+ * Philox key seed + s * 0x9E3779B97F4A7C15, harmonic face means as in
+ * hpb_csr_diffusion2d_f64) held as one block-diagonal CSR matrix, b = A x* with
+ * x* uniform in [-1, 1] (rhs_seed).  Solved together by BiCGStab(2) with one
+ * captured hipGraph per cycle (bicgstabl.h:151-274 recurrences, L = 2, true
+ * residual recomputed every cycle). */
+typedef struct hpb_batch_s* hpb_batch_t;
+int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, double sigma, uint64_t seed,
+                                 uint64_t rhs_seed, hpb_batch_t* out);
+int hpb_batch_info(hpb_batch_t b, int64_t* rows, int64_t* nnz, int* ntiles);
+/* Solve all systems to ||b_s - A_s x_s|| <= rel_tol ||b_s||.  use_graph: replay the
+ * captured cycle (0: launch the same kernels directly); check_every: cycles between
+ * host reads of the active count.  Per-system outputs (host, nsys entries, each
+ * optional): iterations, true relative residual, converged flag; plus wall seconds
+ * and cycles run. */
+int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int use_graph, int check_every,
+                              int* out_iters, double* out_true_rel_resid, int* out_converged, double* out_seconds,
+                              int* out_cycles);
+/* host copies of the solution and of x* (rows entries each, optional) */
+int hpb_batch_solution(hpb_batch_t b, double* host_x, double* host_xstar);
+/* device pointers of the block-diagonal CSR and the right-hand side */
+int hpb_batch_csr(hpb_batch_t b, const int** row_ptr, const int** col_ind, const double** val, const double** rhs);
+int hpb_batch_destroy(hpb_batch_t b);
+
 /* ---- ill-conditioned pair generator -------------------------------------- */
 /* Device-side, seeded GenDot pair (Ogita-Rump-Oishi 2005 Alg. 6.1) replacing
  * generate_vector_pair::generate (generate_vector_pair.hpp:119-182, host-driven,
