@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--op", default="dot2", choices=("dot2", "copy", "nrm2c"),
+                    help="kernel timed for the headline (copy: the plain copy used to calibrate HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the hot path is measured on one MI355X; "
@@ -378,6 +380,17 @@ def main():
     y = H.DeviceArray(n, np.float64, h)
     exact, dd = gendot(H, h, n, args.cond, args.seed, x, y)
     dout = H.DeviceArray(4, np.float64, h)
+    if args.op != "dot2":
+        # profiling aid: time another kernel over the same vectors and stop
+        if args.op == "copy":
+            ms = timed(h, L.hpb_copy_f64, (h.ptr, n, x.ptr, y.ptr), args.steps, args.warmup)
+            nbytes = 16.0 * n
+        else:
+            ms = timed(h, L.hpb_nrm2c_f64_async, (h.ptr, n, x.ptr, dout.ptr), args.steps, args.warmup)
+            nbytes = 8.0 * n
+        print(json.dumps({"op": args.op, "n": n, "ms": round(ms, 5), "gbs": round(nbytes / ms / 1e6, 2),
+                          "algorithmic_bytes": int(nbytes)}), flush=True)
+        return
     log("timing %d Dot2 steps after %d warmup" % (args.steps, args.warmup))
     ms = timed(h, L.hpb_dot2_f64_async, (h.ptr, n, x.ptr, y.ptr, dout.ptr), args.steps, args.warmup)
     res = dout.numpy()
