@@ -261,6 +261,19 @@ int hpb_csr_spmv_lpr_f64(hpb_handle_t h, int lanes_per_row, int compensated, int
                          const int* row_ptr, const int* col_ind, const double* val, double alpha, const double* x,
                          double beta, double* y);
 
+/* ---- MatrixMarket files (host side; csr::matrix_market_reader,
+ *      source/common/csr/matrix_market_reader.h:246-430) ------------------- */
+/* Reads a coordinate real/integer/pattern file (general, symmetric or
+ * skew-symmetric) into zero-based CSR host arrays allocated by the library
+ * (release each with hpb_mm_free).  Entries are stably sorted by (row, column);
+ * duplicates are kept in file order, as in the reference. */
+int hpb_mm_read_csr(const char* path, int* out_rows, int* out_cols, int64_t* out_nnz, int** out_row_ptr,
+                    int** out_col_ind, double** out_val);
+void hpb_mm_free(void* host_ptr);
+/* Writes host CSR arrays as "coordinate real general" with 17 significant digits. */
+int hpb_mm_write_csr(const char* path, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                     const double* val);
+
 /* ---- structured test matrices (the benchmark problems; the reference reads
  *      MatrixMarket files instead, csr/matrix_market_reader.h:246) ---------- */
 /* rows and non-zeros of an nx x ny x nz 7-point (nz == 1: 5-point) stencil with
