@@ -4,7 +4,7 @@ The reference fixes BLOCK_SIZE = 1024 and two elements per thread at compile
 time (source/common/ogita/gpu_reduction_ogita.h:36, gpu_reduction_ogita_impl.cuh:683-694).
 Here the shape is a runtime property of the handle (hpb_set_tuning): threads per
 block, 16-byte vectors per thread per tile, the cap on the grid, and optional
-nontemporal loads.  This module times candidate shapes with HIP events on the
+nontemporal loads (and stores, for the elementwise kernels).  This module times candidate shapes with HIP events on the
 handle's stream, keeps the fastest per (kind, dtype) and stores the table as
 JSON; ``apply`` loads a table into a handle.
 
@@ -47,7 +47,8 @@ def candidates(kind, num_cu):
                 if block == 512 and unroll == 1:
                     continue
                 for mult in (1, 2, 4, 8, 16):
-                    out.append((block, unroll, int(num_cu * mult), 0))
+                    for flags in (0, 1):
+                        out.append((block, unroll, int(num_cu * mult), flags))
     return out
 
 

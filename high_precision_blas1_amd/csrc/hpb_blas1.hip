@@ -68,7 +68,7 @@ template <typename T> struct f_axpbyc {
     }
 };
 
-template <typename T, class F, int NIN, int BLOCK, int UNROLL, bool VEC>
+template <typename T, class F, int NIN, int BLOCK, int UNROLL, bool VEC, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void k_map(size_t n, F f, const T* i0, const T* i1, const T* i2, const T* i3,
                                                T* out)
 {
@@ -89,10 +89,17 @@ __global__ __launch_bounds__(BLOCK) void k_map(size_t n, F f, const T* i0, const
         for (int u = 0; u < UNROLL; ++u) {
             const size_t vi = base + (size_t)u * BLOCK;
             if (vi < nv) {
-                if constexpr (NIN > 0) a0[u] = p0[vi];
-                if constexpr (NIN > 1) a1[u] = p1[vi];
-                if constexpr (NIN > 2) a2[u] = p2[vi];
-                if constexpr (NIN > 3) a3[u] = p3[vi];
+                if constexpr (NT) {
+                    if constexpr (NIN > 0) a0[u] = dev::load_nt<V>(p0 + vi);
+                    if constexpr (NIN > 1) a1[u] = dev::load_nt<V>(p1 + vi);
+                    if constexpr (NIN > 2) a2[u] = dev::load_nt<V>(p2 + vi);
+                    if constexpr (NIN > 3) a3[u] = dev::load_nt<V>(p3 + vi);
+                } else {
+                    if constexpr (NIN > 0) a0[u] = p0[vi];
+                    if constexpr (NIN > 1) a1[u] = p1[vi];
+                    if constexpr (NIN > 2) a2[u] = p2[vi];
+                    if constexpr (NIN > 3) a3[u] = p3[vi];
+                }
             }
         }
 #pragma unroll
@@ -113,7 +120,8 @@ __global__ __launch_bounds__(BLOCK) void k_map(size_t n, F f, const T* i0, const
                     r = f(NIN > 0 ? a0[u] : T(0), NIN > 1 ? a1[u] : T(0), NIN > 2 ? a2[u] : T(0),
                           NIN > 3 ? a3[u] : T(0));
                 }
-                po[vi] = r;
+                if constexpr (NT) dev::store_nt<V>(po + vi, r);
+                else po[vi] = r;
             }
         }
     }
@@ -134,10 +142,13 @@ template <> struct dtid<float> { static constexpr int v = 1; };
 static inline bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 template <typename T, class F, int NIN, int BLOCK, int UNROLL>
-static void map_launch_bu(hipStream_t st, int grid, bool vec, size_t n, const F& f, const T* i0, const T* i1,
-                          const T* i2, const T* i3, T* out)
+static void map_launch_bu(hipStream_t st, int grid, bool vec, bool nt, size_t n, const F& f, const T* i0,
+                          const T* i1, const T* i2, const T* i3, T* out)
 {
-    if (vec)
+    if (vec && nt)
+        hipLaunchKernelGGL((k_map<T, F, NIN, BLOCK, UNROLL, true, true>), dim3(grid), dim3(BLOCK), 0, st, n, f, i0, i1,
+                           i2, i3, out);
+    else if (vec)
         hipLaunchKernelGGL((k_map<T, F, NIN, BLOCK, UNROLL, true>), dim3(grid), dim3(BLOCK), 0, st, n, f, i0, i1, i2,
                            i3, out);
     else
@@ -164,13 +175,14 @@ static int map_entry(hpb_handle_t h, size_t n, const F& f, const T* i0, const T*
     size_t ntiles = (nv + tile - 1) / tile;
     if (ntiles < 1) ntiles = 1;
     const int grid = (int)(ntiles < (size_t)t.grid_max ? ntiles : (size_t)t.grid_max);
+    const bool nt = (t.flags & HPB_TUNE_NONTEMPORAL) != 0;
     if (t.block == 512) {
-        if (t.unroll >= 4) map_launch_bu<T, F, NIN, 512, 4>(h->stream, grid, vec, n, f, i0, i1, i2, i3, out);
-        else map_launch_bu<T, F, NIN, 512, 2>(h->stream, grid, vec, n, f, i0, i1, i2, i3, out);
+        if (t.unroll >= 4) map_launch_bu<T, F, NIN, 512, 4>(h->stream, grid, vec, nt, n, f, i0, i1, i2, i3, out);
+        else map_launch_bu<T, F, NIN, 512, 2>(h->stream, grid, vec, nt, n, f, i0, i1, i2, i3, out);
     } else {
-        if (t.unroll >= 4) map_launch_bu<T, F, NIN, 256, 4>(h->stream, grid, vec, n, f, i0, i1, i2, i3, out);
-        else if (t.unroll == 1) map_launch_bu<T, F, NIN, 256, 1>(h->stream, grid, vec, n, f, i0, i1, i2, i3, out);
-        else map_launch_bu<T, F, NIN, 256, 2>(h->stream, grid, vec, n, f, i0, i1, i2, i3, out);
+        if (t.unroll >= 4) map_launch_bu<T, F, NIN, 256, 4>(h->stream, grid, vec, nt, n, f, i0, i1, i2, i3, out);
+        else if (t.unroll == 1) map_launch_bu<T, F, NIN, 256, 1>(h->stream, grid, vec, nt, n, f, i0, i1, i2, i3, out);
+        else map_launch_bu<T, F, NIN, 256, 2>(h->stream, grid, vec, nt, n, f, i0, i1, i2, i3, out);
     }
     HPB_LAUNCH_CHECK();
     return HPB_OK;

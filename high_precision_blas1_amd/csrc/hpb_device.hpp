@@ -137,5 +137,50 @@ template <> __device__ __forceinline__ float vget<float>(const float4& v, int i)
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
 
+// Nontemporal (streaming) loads and stores; 16-byte vectors stay single
+// dwordx4 instructions with the nt bit.
+template <typename V>
+__device__ __forceinline__ V load_nt(const V* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ double2 load_nt<double2>(const double2* p)
+{
+    double2 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+}
+template <>
+__device__ __forceinline__ float4 load_nt<float4>(const float4* p)
+{
+    float4 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    v.z = __builtin_nontemporal_load(&p->z);
+    v.w = __builtin_nontemporal_load(&p->w);
+    return v;
+}
+template <typename V>
+__device__ __forceinline__ void store_nt(V* p, const V& v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+template <>
+__device__ __forceinline__ void store_nt<double2>(double2* p, const double2& v)
+{
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+template <>
+__device__ __forceinline__ void store_nt<float4>(float4* p, const float4& v)
+{
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+    __builtin_nontemporal_store(v.z, &p->z);
+    __builtin_nontemporal_store(v.w, &p->w);
+}
+
 }  // namespace dev
 }  // namespace hpb

@@ -116,30 +116,6 @@ __device__ __forceinline__ bool take_ticket(unsigned* ticket, unsigned G)
     return last;
 }
 
-template <typename V>
-__device__ __forceinline__ V load_nt(const V* p)
-{
-    return __builtin_nontemporal_load(p);
-}
-template <>
-__device__ __forceinline__ double2 load_nt<double2>(const double2* p)
-{
-    double2 v;
-    v.x = __builtin_nontemporal_load(&p->x);
-    v.y = __builtin_nontemporal_load(&p->y);
-    return v;
-}
-template <>
-__device__ __forceinline__ float4 load_nt<float4>(const float4* p)
-{
-    float4 v;
-    v.x = __builtin_nontemporal_load(&p->x);
-    v.y = __builtin_nontemporal_load(&p->y);
-    v.z = __builtin_nontemporal_load(&p->z);
-    v.w = __builtin_nontemporal_load(&p->w);
-    return v;
-}
-
 // NT: stream the full tiles with nontemporal loads (no reuse is expected).
 template <typename T, int OP, int BLOCK, int UNROLL, bool VEC, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict__ x, const T* __restrict__ y,
@@ -166,8 +142,8 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict_
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             if constexpr (NT) {
-                xa[u] = load_nt<V>(xv + base + (size_t)u * BLOCK);
-                if constexpr (tr::binary) ya[u] = load_nt<V>(yv + base + (size_t)u * BLOCK);
+                xa[u] = dev::load_nt<V>(xv + base + (size_t)u * BLOCK);
+                if constexpr (tr::binary) ya[u] = dev::load_nt<V>(yv + base + (size_t)u * BLOCK);
             } else {
                 xa[u] = xv[base + (size_t)u * BLOCK];
                 if constexpr (tr::binary) ya[u] = yv[base + (size_t)u * BLOCK];

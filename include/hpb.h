@@ -73,7 +73,7 @@ int hpb_memcpy_d2d(hpb_handle_t h, void* dev_dst, const void* dev_src, size_t by
  * (sum, asum, nrm2); dtype: 0 f64, 1 f32.  The autotuner
  * (high_precision_blas1_amd/autotune.py) writes these; BLOCK_SIZE=1024 is a fixed
  * template parameter in the reference (gpu_reduction_ogita.h:36). */
-#define HPB_TUNE_NONTEMPORAL 1 /* reductions: nontemporal loads for the streamed tiles */
+#define HPB_TUNE_NONTEMPORAL 1 /* nontemporal loads (and, elementwise, stores) for the streamed tiles */
 int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max, int flags);
 int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max, int* flags);
 /* Launch shape a reduction of n elements would use (binary: two-input dot, else

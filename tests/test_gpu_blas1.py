@@ -137,3 +137,32 @@ def test_convert(handle):
     H.call("hpb_convert_f32_to_f64", handle.ptr, n, df.ptr, dd.ptr)
     assert np.array_equal(df.numpy(), x.astype(np.float32))
     assert np.array_equal(dd.numpy(), x.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_map_all_launch_shapes(handle, dtype):
+    """Every elementwise launch shape (block, unroll, grid cap, nontemporal
+    flag) gives the same bits; sizes cover ragged tails of every tile width."""
+    r = np.random.default_rng(11)
+    saved = handle.get_tuning(1, dtype)
+    try:
+        for block, unroll in ((256, 1), (256, 2), (256, 4), (512, 2), (512, 4)):
+            for grid in (1, 7, 512):
+                for flags in (0, 1):
+                    handle.set_tuning(1, dtype, block, unroll, grid, flags)
+                    for n in (5, block * unroll * 2 + 3, 300001):
+                        say("map shape", np.dtype(dtype).name, block, unroll, grid, flags, n)
+                        sc = list(r.standard_normal(4).astype(dtype))
+                        ins_h = [r.standard_normal(n).astype(dtype) for _ in range(4)]
+                        ins_d = [H.DeviceArray.from_numpy(a, handle) for a in ins_h]
+                        out = H.DeviceArray(n, dtype, handle)
+                        _call("assign_mul4", handle, dtype, n, sc, ins_d, out)
+                        exp = O.map_op("axpbypcwpdz", n, sc, ins_h, dtype)
+                        assert out.numpy().tobytes() == exp.tobytes()
+                        _call("axpy", handle, dtype, n, sc, ins_d, ins_d[1])
+                        exp = O.map_op("axpy", n, sc, ins_h[:2], dtype)
+                        assert ins_d[1].numpy().tobytes() == exp.tobytes()
+                        for a in ins_d + [out]:
+                            a.free()
+    finally:
+        handle.set_tuning(1, dtype, *saved)
