@@ -364,6 +364,23 @@ int hpb_batch_solution(hpb_batch_t b, double* host_x, double* host_xstar);
 int hpb_batch_csr(hpb_batch_t b, const int** row_ptr, const int** col_ind, const double** val, const double** rhs);
 int hpb_batch_destroy(hpb_batch_t b);
 
+/* ---- exact dot product / sum -------------------------------------------- */
+/* Replaces the exact oracles dot_product_gmp::dot_exact (dot_product_gmp.hpp:108-130,
+ * host GMP) and its CUMP GPU form (dot_product_cump.hpp:134-151,
+ * cump_blas_kernels.cuh:381-444), which return the 512-1024-bit value truncated to
+ * one double.  Here the value is exact (fixed-point long accumulator, bit 0 =
+ * 2^-1074) and returned as out2 = { hi, lo }, hi = round(exact) to nearest even,
+ * lo = round(exact - hi).  Deterministic for any launch shape.
+ * Domain: finite inputs (else HPB_ERR_INVALID); for FP64 dots every nonzero
+ * product satisfies 2^-969 <= |x_i y_i| < 2^960, for sums |x_i| < 2^960 (else
+ * HPB_ERR_UNSUPPORTED); n < 2^38.  The FP32 dot forms the products exactly in FP64. */
+int hpb_exact_dot_f64(hpb_handle_t h, size_t n, const double* x, const double* y, double* out2);
+int hpb_exact_dot_f32(hpb_handle_t h, size_t n, const float* x, const float* y, double* out2);
+int hpb_exact_sum_f64(hpb_handle_t h, size_t n, const double* x, double* out2);
+/* dev_out3 = { hi, lo, flags } in device memory; flags 1 = non-finite input,
+ * 2 = outside the exact range (hi, lo are NaN when flags != 0). */
+int hpb_exact_dot_f64_async(hpb_handle_t h, size_t n, const double* x, const double* y, double* dev_out3);
+
 /* ---- ill-conditioned pair generator -------------------------------------- */
 /* Device-side, seeded GenDot pair (Ogita-Rump-Oishi 2005 Alg. 6.1) replacing
  * generate_vector_pair::generate (generate_vector_pair.hpp:119-182, host-driven,
