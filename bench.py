@@ -112,10 +112,10 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
             ts.append(time.perf_counter() - t0)
         ts.sort()
         core = ts[1:-1]  # drop min and max (test_vector_operations.cpp:44-49)
-        return sum(core) / len(core), len(ts)
+        return sum(core) / len(core), len(ts), core[0], core[-1]
 
-    t_thr, reps_thr = run(fthr, budget_s * 0.6)
-    t_seq, reps_seq = run(fseq, budget_s * 0.4)
+    t_thr, reps_thr, tmin, tmax = run(fthr, budget_s * 0.6)
+    t_seq, reps_seq, _, _ = run(fseq, budget_s * 0.4)
     nbytes = 16.0 * sample
     return {
         "value": round(nbytes / t_thr / 1e9, 3),
@@ -124,6 +124,7 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
         "kind": kind,
         "sample": "FP64 Dot2 over the first %d elements of the benchmark vectors (%d MiB), threaded_reduction "
                   "with %d threads, mean of %d runs without min/max" % (sample, int(nbytes) >> 20, cores, reps_thr),
+        "ms_mean_min_max": [round(1e3 * t_thr, 3), round(1e3 * tmin, 3), round(1e3 * tmax, 3)],
         "single_thread_gbs": round(nbytes / t_seq / 1e9, 3),
         "single_thread_runs": reps_seq,
     }
@@ -189,8 +190,15 @@ def cfg_cond_sweep(H, h, n, log):
         g = d.numpy()
         out = np.zeros(4)
         H.call("hpb_dot_f64", h.ptr, n, x.ptr, y.ptr, out.ctypes.data_as(ctypes.c_void_p))
-        rows.append({"cond_target": cond, "cond_achieved": float(dd[2]), "dot2_gbs": round(16.0 * n / ms / 1e6, 1),
-                     "dot2_rel_err": rel_err(g[2], ex), "naive_rel_err": rel_err(out[2], ex)})
+        row = {"cond_target": cond, "cond_achieved": float(dd[2]), "dot2_gbs": round(16.0 * n / ms / 1e6, 1),
+               "dot2_rel_err": rel_err(g[2], ex), "naive_rel_err": rel_err(out[2], ex)}
+        # device exact dot (long accumulator) against the generator's tracked value
+        ms_x = timed(h, L.hpb_exact_dot_f64_async, (h.ptr, n, x.ptr, y.ptr, d.ptr), 5, 1)
+        e3 = d.numpy()[:3]
+        hi = float(ex)
+        row["exact_dot_gbs"] = round(16.0 * n / ms_x / 1e6, 1)
+        row["exact_dot_equals_generator"] = bool(e3[2] == 0 and e3[0] == hi and e3[1] == float(ex - Fraction(hi)))
+        rows.append(row)
     for a in (x, y, d):
         a.free()
     return rows
@@ -215,6 +223,17 @@ def cfg_blas1(H, h, x, y, n):
     res["axpbyc_gbs"] = round(24.0 * n / ms / 1e6, 1)
     ms = timed(h, L.hpb_dot_f64_async, (h.ptr, n, x.ptr, y.ptr, d.ptr), 20, 3)
     res["dot_naive_gbs"] = round(16.0 * n / ms / 1e6, 1)
+    # end to end: blocking call with the scalar read back to the host, host clock
+    for _ in range(3):
+        H.call("hpb_dot2_f64", h.ptr, n, x.ptr, y.ptr, out.ctypes.data_as(ctypes.c_void_p))
+    ts = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        L.hpb_dot2_f64(h.ptr, n, x.ptr, y.ptr, out.ctypes.data_as(ctypes.c_void_p))
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    res["dot2_end_to_end_ms"] = round(1e3 * sum(ts[1:-1]) / (len(ts) - 2), 4)
+    res["dot2_end_to_end_gbs"] = round(16.0 * n / (sum(ts[1:-1]) / (len(ts) - 2)) / 1e9, 1)
     z.free()
     d.free()
     return res
@@ -362,7 +381,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true")
-    ap.add_argument("--op", default="dot2", choices=("dot2", "copy", "nrm2c"),
+    ap.add_argument("--op", default="dot2", choices=("dot2", "copy", "nrm2c", "exact"),
                     help="kernel timed for the headline (copy: the plain copy used to calibrate HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -384,6 +403,9 @@ def main():
         # profiling aid: time another kernel over the same vectors and stop
         if args.op == "copy":
             ms = timed(h, L.hpb_copy_f64, (h.ptr, n, x.ptr, y.ptr), args.steps, args.warmup)
+            nbytes = 16.0 * n
+        elif args.op == "exact":
+            ms = timed(h, L.hpb_exact_dot_f64_async, (h.ptr, n, x.ptr, y.ptr, dout.ptr), args.steps, args.warmup)
             nbytes = 16.0 * n
         else:
             ms = timed(h, L.hpb_nrm2c_f64_async, (h.ptr, n, x.ptr, dout.ptr), args.steps, args.warmup)

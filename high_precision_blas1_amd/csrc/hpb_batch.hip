@@ -27,6 +27,7 @@
 #include <chrono>
 #include <type_traits>
 #include <cmath>
+#include <cstdio>
 #include <vector>
 
 #include "hpb_device.hpp"
@@ -72,6 +73,7 @@ struct batch_t {
     int* hcount = nullptr;          // pinned
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int verbose = 0;                // print the active count at every check
 };
 enum { VB = 0, VX, VR0, VR1, VR2, VU0, VU1, VU2, VRT, VXS };
 
@@ -561,6 +563,9 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
         cycles += k;
         HPB_HIP(hipMemcpyAsync(B->hcount, B->nactive, sizeof(int), hipMemcpyDeviceToHost, st));
         HPB_HIP(hipStreamSynchronize(st));
+        if (B->verbose)
+            std::fprintf(stderr, "hpb_batch: cycle %d, active systems %d, %.3f s\n", cycles, *B->hcount,
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         if (*B->hcount == 0) break;
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -596,6 +601,13 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     }
     if (out_seconds) *out_seconds = secs;
     if (out_cycles) *out_cycles = cycles;
+    return HPB_OK;
+}
+
+int hpb_batch_set_verbose(hpb_batch_t b, int verbose)
+{
+    HPB_REQUIRE(b, HPB_ERR_INVALID, "hpb_batch_set_verbose: null batch");
+    ((batch_t*)b)->verbose = verbose;
     return HPB_OK;
 }
 

@@ -358,6 +358,9 @@ int hpb_batch_info(hpb_batch_t b, int64_t* rows, int64_t* nnz, int* ntiles);
 int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int use_graph, int check_every,
                               int* out_iters, double* out_true_rel_resid, int* out_converged, double* out_seconds,
                               int* out_cycles);
+/* verbose != 0: print the count of active systems to stderr at every check
+ * (the role of the reference monitor's log output, default_monitor.h:204-214) */
+int hpb_batch_set_verbose(hpb_batch_t b, int verbose);
 /* host copies of the solution and of x* (rows entries each, optional) */
 int hpb_batch_solution(hpb_batch_t b, double* host_x, double* host_xstar);
 /* device pointers of the block-diagonal CSR and the right-hand side */

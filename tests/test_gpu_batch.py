@@ -107,7 +107,9 @@ def test_batch_solve_small(handle, use_graph):
         assert np.linalg.norm(bs - As @ x[sl]) <= 1.5e-10 * np.linalg.norm(bs)
         if s < 6:
             xo, ito, _ = K.bicgstabl(As, bs, np.zeros(n), 2, 1e-10, 4000)
-            assert abs(int(it[s]) - ito) <= 12, (s, it[s], ito)
+            # same recurrences, different rounding (compensated dots, Gram-form MR
+            # part, unnormalised shadow residual): BiCGStab's count drifts a little
+            assert abs(int(it[s]) - ito) <= max(12, 0.1 * ito), (s, it[s], ito)
         row0 += n
     H.call("hpb_batch_destroy", b)
 
