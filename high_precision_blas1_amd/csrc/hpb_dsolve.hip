@@ -1,0 +1,828 @@
+// hpb_dsolve.hip -- device-resident GMRES(m) and BiCGStab(L).
+//
+// Same algorithms and monitor rules as the C++ solver templates (which follow
+// source/numerical_algos/lin_solvers/gmres.h:376-494 and bicgstabl.h:151-274), with
+// every scalar of the iteration kept in device memory, so that no step waits for
+// the host:
+//   * dots, norms and multi-dots write (s, c) pairs to device memory; small
+//     single-thread kernels turn them into the recurrence scalars in FP64 (for
+//     FP32 vectors too: the Hessenberg matrix, the Givens rotations, the back
+//     substitution and BiCGStab's L x L minimal-residual solve are FP64);
+//   * GMRES: one restart cycle (restart residual, m Arnoldi steps with CGS2 --
+//     two fused multi-dot / multi-update passes --, back substitution, x += V y) is
+//     captured once as a hipGraph and replayed; the host reads one flag per cycle;
+//   * BiCGStab(L): one cycle (L BiCG steps, the MR part, the optional true
+//     residual, the norm check) per graph; the host reads the flag every few
+//     cycles;
+//   * convergence, breakdown and the iteration cap raise device flags; every
+//     kernel of a graph tests its flag on entry, so replays after the end do no
+//     work and leave x untouched.
+// MR part of BiCGStab(L): the normal equations of min ||r_0 - sum_j g_j r_j||,
+// G g = c with G_ij = (r_i, r_j), c_i = (r_i, r_0) (1 <= i, j <= L), are formed by
+// fused compensated multi-dots and solved by Cholesky in FP64; in exact arithmetic
+// this is the reference's modified Gram-Schmidt solution (bicgstabl.h:224-245),
+// and x, r_0, u_0 are then updated in one pass.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hpb_device.hpp"
+#include "hpb_internal.hpp"
+
+namespace hpb {
+namespace {
+
+constexpr int DB = 256;        // threads per block of the vector kernels
+constexpr int MAXB = 256;      // GMRES basis vectors (m + 1 <= 256)
+constexpr int MAXL = 8;        // BiCGStab(L) basis in the device path
+
+struct dstate {
+    double tol, bnorm, rnorm;
+    double rho, alpha, omega;
+    int iters, max_iters, min_iters, cycles;
+    int done, stop, converged, breakdown, invalid;
+    int cycle_len, hist_len, hist_cap, recalc_freq, skip_recalc;
+    int early_exit, basis;
+};
+
+template <typename T>
+__device__ __forceinline__ double dval(const T* o4)
+{
+    return (double)o4[0] + (double)o4[1];
+}
+
+__device__ __forceinline__ bool finite(double v) { return v == v && v - v == 0.0; }
+
+__device__ void record(dstate* st, double* hist, double rn)
+{
+    if (hist && st->hist_len < st->hist_cap) {
+        hist[2 * st->hist_len] = (double)st->iters;
+        hist[2 * st->hist_len + 1] = st->bnorm > 0.0 ? rn / st->bnorm : rn;
+        st->hist_len++;
+    }
+}
+
+// monitor check (default_monitor.h:200-227): converged when the residual is a
+// valid number, <= tol and at least min_iters steps ran; finished also at the cap
+__device__ void monitor_check(dstate* st, double* hist, double rn)
+{
+    st->rnorm = rn;
+    record(st, hist, rn);
+    if (!finite(rn)) {
+        st->invalid = 1;
+        st->done = 1;
+        return;
+    }
+    if (rn <= st->tol && st->iters >= st->min_iters) {
+        st->converged = 1;
+        st->done = 1;
+        return;
+    }
+    if (st->iters >= st->max_iters) st->done = 1;
+}
+
+// ---------------------------------------------------------------- common kernels
+
+template <typename T>
+__global__ void k_init(dstate* st, const T* nb4, double rel_tol, double abs_tol)
+{
+    const double bn = sqrt(dval(nb4));
+    st->bnorm = bn;
+    st->tol = abs_tol + rel_tol * bn;
+}
+
+// v := s v, s read from device memory
+template <typename T>
+__global__ __launch_bounds__(DB) void k_scale(size_t n, T* __restrict__ v, const T* __restrict__ s,
+                                              const int* __restrict__ skip)
+{
+    if (*skip) return;
+    const T a = *s;
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) v[i] = a * v[i];
+}
+
+// y := s x
+template <typename T>
+__global__ __launch_bounds__(DB) void k_scale_copy(size_t n, const T* __restrict__ x, T* __restrict__ y,
+                                                   const T* __restrict__ s, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    const T a = *s;
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) y[i] = a * x[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(DB) void k_fill(size_t n, T* __restrict__ v, T a)
+{
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) v[i] = a;
+}
+
+// ---------------------------------------------------------------- GMRES kernels
+
+// restart: beta = ||r||, monitor check on the true residual, g = beta e_1
+template <typename T>
+__global__ void k_gm_start(dstate* st, const T* n4, double* g, int m, T* scal, double* hist)
+{
+    if (st->done) {
+        st->stop = 1;
+        return;
+    }
+    const double beta = sqrt(dval(n4));
+    monitor_check(st, hist, beta);
+    if (st->done) {
+        st->stop = 1;
+        return;
+    }
+    st->stop = 0;
+    st->cycle_len = 0;
+    st->cycles++;
+    g[0] = beta;
+    for (int i = 1; i <= m; ++i) g[i] = 0.0;
+    *scal = (T)(1.0 / beta);
+}
+
+// Hessenberg column j from a multi-dot pass; coef = -h for the multi-update
+template <typename T>
+__global__ __launch_bounds__(DB) void k_gm_coef(int j, const T* __restrict__ d4, double* __restrict__ H, int ldh,
+                                                T* __restrict__ coef, int pass, const int* __restrict__ stop)
+{
+    if (*stop) return;
+    for (int i = threadIdx.x; i <= j; i += DB) {
+        const double h = dval(d4 + 4 * i);
+        if (pass == 0) H[(size_t)j * ldh + i] = h;
+        else H[(size_t)j * ldh + i] += h;
+        coef[i] = (T)(-h);
+    }
+}
+
+__device__ __forceinline__ void apply_rot(double& dx, double& dy, double cs, double sn)
+{
+    const double t = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = t;
+}
+
+// new column j: previous rotations, new rotation, residual estimate (gmres.h:217-274)
+template <typename T>
+__global__ void k_gm_givens(int j, int m, const T* n4, double* H, int ldh, double* cs, double* sn, double* g,
+                            T* scal, dstate* st, double* hist)
+{
+    if (st->stop) return;
+    double* Hj = H + (size_t)j * ldh;
+    const double hn = sqrt(dval(n4));
+    Hj[j + 1] = hn;
+    for (int k = 0; k < j; ++k) apply_rot(Hj[k], Hj[k + 1], cs[k], sn[k]);
+    double c, s;
+    const double dx = Hj[j], dy = Hj[j + 1];
+    if (dy == 0.0) {
+        c = 1.0;
+        s = 0.0;
+    } else if (fabs(dy) > fabs(dx)) {
+        const double t = dx / dy;
+        s = 1.0 / sqrt(1.0 + t * t);
+        c = t * s;
+    } else {
+        const double t = dy / dx;
+        c = 1.0 / sqrt(1.0 + t * t);
+        s = t * c;
+    }
+    cs[j] = c;
+    sn[j] = s;
+    apply_rot(Hj[j], Hj[j + 1], c, s);
+    Hj[j + 1] = 0.0;
+    apply_rot(g[j], g[j + 1], c, s);
+    st->iters++;
+    st->cycle_len = j + 1;
+    const double est = fabs(g[j + 1]);
+    st->rnorm = est;
+    *scal = hn != 0.0 ? (T)(1.0 / hn) : T(0);
+    if (hn == 0.0 || !finite(est) || (st->early_exit && est <= st->tol) || j + 1 >= m || st->iters >= st->max_iters)
+        st->stop = 1;
+}
+
+// y = R^{-1} g over the steps of this cycle (gmres.h:204-215); y[k..m) = 0
+template <typename T>
+__global__ void k_gm_backsub(int m, const double* H, int ldh, double* g, T* y, const dstate* st)
+{
+    if (st->done) return;
+    const int k = st->cycle_len;
+    for (int j = k - 1; j >= 0; --j) {
+        g[j] /= H[(size_t)j * ldh + j];
+        for (int i = j - 1; i >= 0; --i) g[i] -= H[(size_t)j * ldh + i] * g[j];
+    }
+    for (int j = 0; j < m; ++j) y[j] = j < k ? (T)g[j] : T(0);
+}
+
+template <typename T>
+struct basis_pack {
+    const T* p[MAXB];
+};
+
+// x += sum_{j < cycle_len} y_j V_j (compensated: Dot2 over the terms of each element)
+template <typename T, bool COMP>
+__global__ __launch_bounds__(DB) void k_gm_xupd(size_t n, basis_pack<T> V, const T* __restrict__ y,
+                                                T* __restrict__ x, const dstate* __restrict__ st)
+{
+    if (st->done) return;
+    const int k = st->cycle_len;
+    __shared__ T sy[MAXB];
+    for (int j = threadIdx.x; j < k; j += DB) sy[j] = y[j];
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        T s = x[i], c = T(0);
+        for (int j = 0; j < k; ++j) {
+            if (COMP) dev::dot2_step<T>(s, c, sy[j], V.p[j][i]);
+            else s = dev::fma_rn<T>(sy[j], V.p[j][i], s);
+        }
+        x[i] = COMP ? s + c : s;
+    }
+}
+
+// ---------------------------------------------------------------- BiCGStab(L) kernels
+
+template <typename T>
+struct bs_pack {
+    T* r[MAXL + 1];
+    T* u[MAXL + 1];
+};
+
+template <typename T>
+__global__ void k_bs_init(dstate* st, const T* n4, T* scal, double* hist)
+{
+    const double rn = sqrt(dval(n4));
+    st->rho = 1.0;
+    st->alpha = 0.0;
+    st->omega = 1.0;
+    monitor_check(st, hist, rn);
+    *scal = rn != 0.0 ? (T)(1.0 / rn) : T(1);
+}
+
+// rho1 = (r_j, rt); beta = alpha rho1 / rho (bicgstabl.h:199-206)
+template <typename T>
+__global__ void k_bs_beta(dstate* st, const T* d4, T* coef, int first)
+{
+    if (st->done) return;
+    double rho = st->rho;
+    if (first) rho = -st->omega * rho;
+    if (rho == 0.0 || !finite(rho)) {
+        st->breakdown = 1;
+        st->done = 1;
+        return;
+    }
+    const double rho1 = dval(d4);
+    const double beta = st->alpha * rho1 / rho;
+    st->rho = rho1;
+    coef[0] = (T)beta;
+}
+
+// alpha = rho / (u_{j+1}, rt)
+template <typename T>
+__global__ void k_bs_alpha(dstate* st, const T* d4, T* coef)
+{
+    if (st->done) return;
+    const double sg = dval(d4);
+    if (sg == 0.0 || !finite(sg)) {
+        st->breakdown = 1;
+        st->done = 1;
+        return;
+    }
+    st->alpha = st->rho / sg;
+    coef[1] = (T)st->alpha;
+}
+
+// u_i = r_i - beta u_i, i = 0..j
+template <typename T>
+__global__ __launch_bounds__(DB) void k_bs_upd_u(size_t n, int j, bs_pack<T> P, const T* __restrict__ coef,
+                                                 const int* __restrict__ skip)
+{
+    if (*skip) return;
+    const T beta = coef[0];
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB)
+        for (int q = 0; q <= j; ++q) P.u[q][i] = dev::fma_rn<T>(-beta, P.u[q][i], P.r[q][i]);
+}
+
+// r_i -= alpha u_{i+1}, i = 0..j;  x += alpha u_0
+template <typename T>
+__global__ __launch_bounds__(DB) void k_bs_upd_r(size_t n, int j, bs_pack<T> P, T* __restrict__ x,
+                                                 const T* __restrict__ coef, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    const T alpha = coef[1];
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        for (int q = 0; q <= j; ++q) P.r[q][i] = dev::fma_rn<T>(-alpha, P.u[q + 1][i], P.r[q][i]);
+        x[i] = dev::fma_rn<T>(alpha, P.u[0][i], x[i]);
+    }
+}
+
+// MR part: gram[k] holds (r_i, r_k), i = 0..k, as 4-word (s, c, ...) entries.
+// Solves G g = c by Cholesky in FP64; coef[2 + j] = g_{j+1}.
+template <typename T>
+__global__ void k_bs_mr(dstate* st, int L, const T* gram, int gstride, T* coef)
+{
+    if (st->done) return;
+    double G[MAXL][MAXL], c[MAXL], y[MAXL];
+    for (int k = 1; k <= L; ++k) {
+        const T* gk = gram + (size_t)k * gstride;   // entries (r_i, r_k), i = 0..k
+        c[k - 1] = dval(gk);
+        for (int i = 1; i <= k; ++i) {
+            const double v = dval(gk + 4 * i);
+            G[i - 1][k - 1] = v;
+            G[k - 1][i - 1] = v;
+        }
+    }
+    // Cholesky G = R^T R (upper R stored in G)
+    for (int j = 0; j < L; ++j) {
+        double d = G[j][j];
+        for (int k = 0; k < j; ++k) d -= G[k][j] * G[k][j];
+        if (!(d > 0.0) || !finite(d)) {
+            st->breakdown = 1;
+            st->done = 1;
+            return;
+        }
+        d = sqrt(d);
+        G[j][j] = d;
+        for (int i = j + 1; i < L; ++i) {
+            double v = G[j][i];
+            for (int k = 0; k < j; ++k) v -= G[k][j] * G[k][i];
+            G[j][i] = v / d;
+        }
+    }
+    for (int i = 0; i < L; ++i) {   // R^T y = c
+        double v = c[i];
+        for (int k = 0; k < i; ++k) v -= G[k][i] * y[k];
+        y[i] = v / G[i][i];
+    }
+    for (int i = L - 1; i >= 0; --i) {   // R g = y
+        double v = y[i];
+        for (int k = i + 1; k < L; ++k) v -= G[i][k] * y[k];
+        y[i] = v / G[i][i];
+    }
+    st->omega = y[L - 1];
+    for (int j = 0; j < L; ++j) coef[2 + j] = (T)y[j];
+}
+
+// x += sum_j g_j r_{j-1};  r_0 -= sum_j g_j r_j;  u_0 -= sum_j g_j u_j  (j = 1..L)
+template <typename T, bool COMP>
+__global__ __launch_bounds__(DB) void k_bs_upd_mr(size_t n, int L, bs_pack<T> P, T* __restrict__ x,
+                                                  const T* __restrict__ coef, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    __shared__ T gm[MAXL];
+    if (threadIdx.x < (unsigned)L) gm[threadIdx.x] = coef[2 + threadIdx.x];
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        T sx = x[i], cx = T(0), sr = P.r[0][i], cr = T(0), su = P.u[0][i], cu = T(0);
+        T rprev = sr;
+        for (int j = 1; j <= L; ++j) {
+            const T g = gm[j - 1];
+            const T rj = P.r[j][i];
+            const T uj = P.u[j][i];
+            if (COMP) {
+                dev::dot2_step<T>(sx, cx, g, rprev);
+                dev::dot2_step<T>(sr, cr, -g, rj);
+                dev::dot2_step<T>(su, cu, -g, uj);
+            } else {
+                sx = dev::fma_rn<T>(g, rprev, sx);
+                sr = dev::fma_rn<T>(-g, rj, sr);
+                su = dev::fma_rn<T>(-g, uj, su);
+            }
+            rprev = rj;
+        }
+        x[i] = COMP ? sx + cx : sx;
+        P.r[0][i] = COMP ? sr + cr : sr;
+        P.u[0][i] = COMP ? su + cu : su;
+    }
+}
+
+// true residual every recalc_freq cycles (bicgstabl.h:247-263)
+__global__ void k_bs_recalc_flag(dstate* st)
+{
+    st->skip_recalc = 1;
+    if (st->done) return;
+    const int f = st->recalc_freq;
+    if (f > 0 && ((st->iters / st->basis + 1) % f) == 0) st->skip_recalc = 0;
+}
+
+template <typename T>
+__global__ void k_bs_check(dstate* st, const T* n4, double* hist)
+{
+    if (st->done) return;
+    st->iters += st->basis;
+    st->cycles++;
+    monitor_check(st, hist, sqrt(dval(n4)));
+}
+
+// ---------------------------------------------------------------- host side
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+struct dbuf {
+    T* p = nullptr;
+    dbuf() = default;
+    dbuf(const dbuf&) = delete;
+    dbuf& operator=(const dbuf&) = delete;
+    dbuf(dbuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+    ~dbuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t count)
+    {
+        const hipError_t e = hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
+        if (e != hipSuccess) return hip_status(e, __FILE__, __LINE__, "device solver workspace");
+        return HPB_OK;
+    }
+};
+
+struct graph_guard {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    graph_guard() = default;
+    graph_guard(const graph_guard&) = delete;
+    graph_guard& operator=(const graph_guard&) = delete;
+    ~graph_guard()
+    {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+unsigned vgrid(const hpb_handle_s* h, size_t n)
+{
+    size_t g = (n + DB - 1) / DB;
+    const size_t cap = (size_t)(h->num_cu > 0 ? h->num_cu : 256) * 8;
+    if (g > cap) g = cap;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+#define DS_CHECK(call)                  \
+    do {                                \
+        const int _rc = (call);         \
+        if (_rc != HPB_OK) return _rc;  \
+    } while (0)
+
+template <typename T>
+struct problem {
+    hpb_handle_t h;
+    int rows;
+    int64_t nnz;
+    const int* rp;
+    const int* ci;
+    const T* val;
+    const T* b;
+    T* x;
+};
+
+template <typename T>
+int end_capture(hipStream_t st, graph_guard& gg, int rc)
+{
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != HPB_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_status(e, __FILE__, __LINE__, "hipStreamEndCapture");
+    gg.g = g;
+    HPB_HIP(hipGraphInstantiate(&gg.e, gg.g, nullptr, nullptr, 0));
+    return HPB_OK;
+}
+
+// multi-dot / multi-update over more than 64 columns: chunks of 64
+template <typename T>
+int multidot_chunks(hpb_handle_t h, int k, T* const* V, size_t n, const T* w, bool comp, T* d4, const int* skip)
+{
+    for (int c0 = 0; c0 < k; c0 += HPB_MAX_MULTIDOT) {
+        const int kc = k - c0 < HPB_MAX_MULTIDOT ? k - c0 : HPB_MAX_MULTIDOT;
+        DS_CHECK(multidot_async<T>(h, kc, (const T* const*)(V + c0), n, w, comp, d4 + 4 * c0, skip));
+    }
+    return HPB_OK;
+}
+template <typename T>
+int multiaxpy_chunks(hpb_handle_t h, int k, const T* coef, T* const* V, size_t n, T* y, bool comp, const int* skip)
+{
+    for (int c0 = 0; c0 < k; c0 += HPB_MAX_MULTIDOT) {
+        const int kc = k - c0 < HPB_MAX_MULTIDOT ? k - c0 : HPB_MAX_MULTIDOT;
+        DS_CHECK(multiaxpy_devcoef<T>(h, kc, coef + c0, (const T* const*)(V + c0), n, y, comp, skip));
+    }
+    return HPB_OK;
+}
+
+template <typename T>
+int read_state(hpb_handle_t h, const dstate* d, dstate* out)
+{
+    HPB_HIP(hipMemcpyAsync(h->hres, d, sizeof(dstate), hipMemcpyDeviceToHost, h->stream));
+    HPB_HIP(hipStreamSynchronize(h->stream));
+    std::memcpy(out, h->hres, sizeof(dstate));
+    return HPB_OK;
+}
+
+template <typename T>
+int init_state(hpb_handle_t h, dstate* d, const hpb_solver_params& p, int basis, int hist_cap)
+{
+    dstate s{};
+    s.max_iters = p.max_iters;
+    s.min_iters = p.min_iters;
+    s.recalc_freq = p.resid_recalc_freq;
+    s.early_exit = p.early_exit;
+    s.basis = basis;
+    s.hist_cap = hist_cap;
+    std::memcpy(h->hres, &s, sizeof s);
+    HPB_HIP(hipMemcpyAsync(d, h->hres, sizeof s, hipMemcpyHostToDevice, h->stream));
+    HPB_HIP(hipStreamSynchronize(h->stream));
+    return HPB_OK;
+}
+
+// One solver instance: buffers and the captured cycle graph live as long as the
+// object, so repeated solves with the same operands (the inner solves of
+// iterative refinement) replay the graph without allocating or capturing again.
+template <typename T>
+class dsolver : public dsolver_base {
+public:
+    problem<T> P{};
+    hpb_solver_params prm{};
+    int hcap = 0;
+
+    int setup(const problem<T>& pr, const hpb_solver_params& p, int hist_cap)
+    {
+        P = pr;
+        prm = p;
+        hcap = hist_cap;
+        DS_CHECK(dst.alloc(1));
+        if (hcap) DS_CHECK(dhist.alloc(2 * (size_t)hcap));
+        return p.solver == HPB_SOLVER_GMRES ? setup_gmres() : setup_bicgstabl();
+    }
+    bool matches(const problem<T>& pr, const hpb_solver_params& p, int hist_cap) const
+    {
+        const bool same_problem = pr.h == P.h && pr.rows == P.rows && pr.nnz == P.nnz && pr.rp == P.rp &&
+                                  pr.ci == P.ci && pr.val == P.val && pr.b == P.b && pr.x == P.x;
+        const bool same_params = p.solver == prm.solver && p.restart == prm.restart && p.basis == prm.basis &&
+                                 p.high_precision == prm.high_precision &&
+                                 p.compensated_spmv == prm.compensated_spmv;
+        return same_problem && same_params && hist_cap == hcap;
+    }
+    // monitor settings (tolerances, iteration caps, early exit, recalculation)
+    // are read from p on every run; the captured graph does not depend on them
+    int run(const hpb_solver_params& p, dstate* out)
+    {
+        prm.max_iters = p.max_iters;
+        prm.min_iters = p.min_iters;
+        prm.rel_tol = p.rel_tol;
+        prm.abs_tol = p.abs_tol;
+        prm.early_exit = p.early_exit;
+        prm.resid_recalc_freq = p.resid_recalc_freq;
+        DS_CHECK(init_state<T>(P.h, dst.p, prm, prm.solver == HPB_SOLVER_GMRES ? 1 : prm.basis, hcap));
+        return prm.solver == HPB_SOLVER_GMRES ? run_gmres(out) : run_bicgstabl(out);
+    }
+    const double* history() const { return dhist.p; }
+
+private:
+    dbuf<dstate> dst;
+    dbuf<double> dhist;
+    graph_guard gg;
+    std::vector<dbuf<T>> Vb, Ub;
+    std::vector<T*> V;
+    dbuf<double> H, cs, sn, g;
+    dbuf<T> d4, n4, coef, y, scal, rt, da, db, gram;
+    int gstride = 0;
+
+    double* hist() { return hcap ? dhist.p : nullptr; }
+
+    int setup_gmres()
+    {
+        hpb_handle_t h = P.h;
+        hipStream_t st = h->stream;
+        const size_t n = (size_t)P.rows;
+        const int m = prm.restart;
+        const bool comp = prm.high_precision != 0, cspmv = prm.compensated_spmv != 0;
+        const int ldh = m + 1;
+        Vb.resize(m + 1);
+        V.assign(m + 1, nullptr);
+        for (int j = 0; j <= m; ++j) {
+            DS_CHECK(Vb[j].alloc(n));
+            V[j] = Vb[j].p;
+        }
+        DS_CHECK(H.alloc((size_t)ldh * m));
+        DS_CHECK(cs.alloc(m));
+        DS_CHECK(sn.alloc(m));
+        DS_CHECK(g.alloc(m + 1));
+        DS_CHECK(d4.alloc(4 * (size_t)(m + 1)));
+        DS_CHECK(n4.alloc(4));
+        DS_CHECK(coef.alloc(m + 1));
+        DS_CHECK(y.alloc(m));
+        DS_CHECK(scal.alloc(1));
+        dstate* S = dst.p;
+        const int* done = &S->done;
+        const int* stop = &S->stop;
+        basis_pack<T> bp{};
+        for (int j = 0; j < m; ++j) bp.p[j] = V[j];
+        double* hp = hist();
+        HPB_HIP(hipStreamSynchronize(st));
+        HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = [&]() -> int {
+            DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, P.x, P.b, V[0], cspmv, done));
+            DS_CHECK(nrm2_async<T>(h, n, V[0], n4.p, comp, done));
+            hipLaunchKernelGGL((k_gm_start<T>), dim3(1), dim3(1), 0, st, S, n4.p, g.p, m, scal.p, hp);
+            hipLaunchKernelGGL((k_scale<T>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, V[0], scal.p, stop);
+            for (int j = 0; j < m; ++j) {
+                DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, V[j], nullptr, V[j + 1], cspmv, stop));
+                for (int pass = 0; pass < 2; ++pass) {
+                    DS_CHECK(multidot_chunks<T>(h, j + 1, V.data(), n, V[j + 1], comp, d4.p, stop));
+                    hipLaunchKernelGGL((k_gm_coef<T>), dim3(1), dim3(DB), 0, st, j, d4.p, H.p, ldh, coef.p, pass,
+                                       stop);
+                    DS_CHECK(multiaxpy_chunks<T>(h, j + 1, coef.p, V.data(), n, V[j + 1], comp, stop));
+                }
+                DS_CHECK(nrm2_async<T>(h, n, V[j + 1], n4.p, comp, stop));
+                hipLaunchKernelGGL((k_gm_givens<T>), dim3(1), dim3(1), 0, st, j, m, n4.p, H.p, ldh, cs.p, sn.p, g.p,
+                                   scal.p, S, hp);
+                hipLaunchKernelGGL((k_scale<T>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, V[j + 1], scal.p, stop);
+            }
+            hipLaunchKernelGGL((k_gm_backsub<T>), dim3(1), dim3(1), 0, st, m, H.p, ldh, g.p, y.p, S);
+            if (comp)
+                hipLaunchKernelGGL((k_gm_xupd<T, true>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, bp, y.p, P.x, S);
+            else
+                hipLaunchKernelGGL((k_gm_xupd<T, false>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, bp, y.p, P.x, S);
+            return HPB_OK;
+        }();
+        return end_capture<T>(st, gg, rc);
+    }
+
+    int run_gmres(dstate* out)
+    {
+        hpb_handle_t h = P.h;
+        hipStream_t st = h->stream;
+        const bool comp = prm.high_precision != 0;
+        DS_CHECK(nrm2_async<T>(h, (size_t)P.rows, P.b, n4.p, comp, nullptr));
+        hipLaunchKernelGGL((k_init<T>), dim3(1), dim3(1), 0, st, dst.p, n4.p, prm.rel_tol, prm.abs_tol);
+        HPB_LAUNCH_CHECK();
+        // every replay starts with the restart residual and the monitor check,
+        // so the replay after the last useful cycle ends the solve
+        const int max_cycles = prm.max_iters + 2;
+        for (int c = 0; c < max_cycles; ++c) {
+            HPB_HIP(hipGraphLaunch(gg.e, st));
+            DS_CHECK(read_state<T>(h, dst.p, out));
+            if (out->done) break;
+        }
+        return HPB_OK;
+    }
+
+    int setup_bicgstabl()
+    {
+        hpb_handle_t h = P.h;
+        hipStream_t st = h->stream;
+        const size_t n = (size_t)P.rows;
+        const int L = prm.basis;
+        const bool comp = prm.high_precision != 0, cspmv = prm.compensated_spmv != 0;
+        Vb.resize(L + 1);
+        Ub.resize(L + 1);
+        V.assign(L + 1, nullptr);
+        bs_pack<T> bp{};
+        for (int j = 0; j <= L; ++j) {
+            DS_CHECK(Vb[j].alloc(n));
+            DS_CHECK(Ub[j].alloc(n));
+            bp.r[j] = Vb[j].p;
+            bp.u[j] = Ub[j].p;
+            V[j] = Vb[j].p;
+        }
+        gstride = 4 * (L + 1);
+        DS_CHECK(rt.alloc(n));
+        DS_CHECK(n4.alloc(4));
+        DS_CHECK(da.alloc(4));
+        DS_CHECK(db.alloc(4));
+        DS_CHECK(gram.alloc((size_t)gstride * (L + 1)));
+        DS_CHECK(coef.alloc(2 + MAXL));
+        DS_CHECK(scal.alloc(1));
+        dstate* S = dst.p;
+        const int* done = &S->done;
+        const unsigned vg = vgrid(h, n);
+        double* hp = hist();
+        HPB_HIP(hipStreamSynchronize(st));
+        HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = [&]() -> int {
+            for (int j = 0; j < L; ++j) {
+                DS_CHECK(dot2_async<T>(h, n, V[j], rt.p, da.p, comp, done));
+                hipLaunchKernelGGL((k_bs_beta<T>), dim3(1), dim3(1), 0, st, S, da.p, coef.p, j == 0 ? 1 : 0);
+                hipLaunchKernelGGL((k_bs_upd_u<T>), dim3(vg), dim3(DB), 0, st, n, j, bp, coef.p, done);
+                DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, Ub[j].p, nullptr, Ub[j + 1].p, cspmv,
+                                      done));
+                DS_CHECK(dot2_async<T>(h, n, Ub[j + 1].p, rt.p, db.p, comp, done));
+                hipLaunchKernelGGL((k_bs_alpha<T>), dim3(1), dim3(1), 0, st, S, db.p, coef.p);
+                hipLaunchKernelGGL((k_bs_upd_r<T>), dim3(vg), dim3(DB), 0, st, n, j, bp, P.x, coef.p, done);
+                DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, V[j], nullptr, V[j + 1], cspmv, done));
+            }
+            for (int k = 1; k <= L; ++k)
+                DS_CHECK(multidot_async<T>(h, k + 1, (const T* const*)V.data(), n, V[k], comp,
+                                           gram.p + (size_t)k * gstride, done));
+            hipLaunchKernelGGL((k_bs_mr<T>), dim3(1), dim3(1), 0, st, S, L, gram.p, gstride, coef.p);
+            if (comp)
+                hipLaunchKernelGGL((k_bs_upd_mr<T, true>), dim3(vg), dim3(DB), 0, st, n, L, bp, P.x, coef.p, done);
+            else
+                hipLaunchKernelGGL((k_bs_upd_mr<T, false>), dim3(vg), dim3(DB), 0, st, n, L, bp, P.x, coef.p, done);
+            hipLaunchKernelGGL(k_bs_recalc_flag, dim3(1), dim3(1), 0, st, S);
+            DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, P.x, P.b, V[0], cspmv, &S->skip_recalc));
+            DS_CHECK(nrm2_async<T>(h, n, V[0], n4.p, comp, done));
+            hipLaunchKernelGGL((k_bs_check<T>), dim3(1), dim3(1), 0, st, S, n4.p, hp);
+            return HPB_OK;
+        }();
+        return end_capture<T>(st, gg, rc);
+    }
+
+    int run_bicgstabl(dstate* out)
+    {
+        hpb_handle_t h = P.h;
+        hipStream_t st = h->stream;
+        const size_t n = (size_t)P.rows;
+        const int L = prm.basis;
+        const bool comp = prm.high_precision != 0, cspmv = prm.compensated_spmv != 0;
+        const unsigned vg = vgrid(h, n);
+        dstate* S = dst.p;
+        // ||b||, tol, r_0 = b - A x, r~ = r_0 / ||r_0||, u_0 = 0
+        DS_CHECK(nrm2_async<T>(h, n, P.b, n4.p, comp, nullptr));
+        hipLaunchKernelGGL((k_init<T>), dim3(1), dim3(1), 0, st, S, n4.p, prm.rel_tol, prm.abs_tol);
+        DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, P.x, P.b, V[0], cspmv, nullptr));
+        DS_CHECK(nrm2_async<T>(h, n, V[0], n4.p, comp, nullptr));
+        hipLaunchKernelGGL((k_bs_init<T>), dim3(1), dim3(1), 0, st, S, n4.p, scal.p, hist());
+        hipLaunchKernelGGL((k_scale_copy<T>), dim3(vg), dim3(DB), 0, st, n, V[0], rt.p, scal.p, &S->invalid);
+        hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(DB), 0, st, n, Ub[0].p, T(0));
+        HPB_LAUNCH_CHECK();
+        DS_CHECK(read_state<T>(h, S, out));
+        if (out->done) return HPB_OK;
+        const int max_cycles = prm.max_iters / L + 2;
+        const int every = 4;
+        for (int c = 0; c < max_cycles; c += every) {
+            for (int k = 0; k < every; ++k) HPB_HIP(hipGraphLaunch(gg.e, st));
+            DS_CHECK(read_state<T>(h, S, out));
+            if (out->done) break;
+        }
+        return HPB_OK;
+    }
+};
+
+}  // namespace
+
+// Device-resident solve of A x = b (vectors of type T, scalars FP64).  Fills res
+// (iters, converged, breakdown, resid_norm, rhs_norm, history_len) and history.
+// cache (optional): a solver object kept by the caller between calls with the
+// same operands; *cache is replaced when the operands or parameters change and
+// must be released with dsolve_release.
+template <typename T>
+int dsolve(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, const T* val, const T* b, T* x,
+           const hpb_solver_params& p, hpb_solver_result* res, double* history, int history_cap,
+           dsolver_base** cache)
+{
+    HPB_REQUIRE(p.solver != HPB_SOLVER_GMRES || (p.restart >= 1 && p.restart < MAXB), HPB_ERR_INVALID,
+                "device solver: GMRES restart must be in [1, 255]");
+    HPB_REQUIRE(p.solver != HPB_SOLVER_BICGSTABL || (p.basis >= 1 && p.basis <= MAXL), HPB_ERR_UNSUPPORTED,
+                "device solver: BiCGStab(L) needs 1 <= L <= 8");
+    const int hcap = (history && history_cap > 0) ? history_cap : 0;
+    problem<T> P{h, rows, nnz, rp, ci, val, b, x};
+    dsolver<T>* ds = nullptr;
+    dsolver<T> local;
+    if (cache && *cache) {
+        ds = static_cast<dsolver<T>*>(*cache);
+        if (!ds->matches(P, p, hcap)) {
+            delete ds;
+            *cache = nullptr;
+            ds = nullptr;
+        }
+    }
+    if (!ds) {
+        ds = cache ? new dsolver<T>() : &local;
+        const int rc = ds->setup(P, p, hcap);
+        if (rc != HPB_OK) {
+            if (cache) delete ds;
+            return rc;
+        }
+        if (cache) *cache = ds;
+    }
+    dstate out{};
+    DS_CHECK(ds->run(p, &out));
+    res->iters = out.iters;
+    res->converged = out.converged;
+    res->breakdown = out.breakdown;
+    res->resid_norm = out.rnorm;
+    res->rhs_norm = out.bnorm;
+    res->history_len = out.hist_len;
+    if (hcap && out.hist_len > 0) {
+        HPB_HIP(hipMemcpyAsync(history, ds->history(), sizeof(double) * 2 * (size_t)out.hist_len,
+                               hipMemcpyDeviceToHost, h->stream));
+        HPB_HIP(hipStreamSynchronize(h->stream));
+    }
+    return HPB_OK;
+}
+
+void dsolve_release(dsolver_base* cache) { delete cache; }
+
+template int dsolve<double>(hpb_handle_t, int, int64_t, const int*, const int*, const double*, const double*,
+                            double*, const hpb_solver_params&, hpb_solver_result*, double*, int, dsolver_base**);
+template int dsolve<float>(hpb_handle_t, int, int64_t, const int*, const int*, const float*, const float*, float*,
+                           const hpb_solver_params&, hpb_solver_result*, double*, int, dsolver_base**);
+
+}  // namespace hpb

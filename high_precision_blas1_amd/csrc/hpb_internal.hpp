@@ -44,6 +44,35 @@ namespace hpb {
 void set_error(const char* file, int line, const std::string& msg);
 int hip_status(hipError_t e, const char* file, int line, const char* what);
 
+// Asynchronous launchers used by the device-resident solvers.  Results stay in
+// device memory; skip (device int, may be null) makes the launch a no-op while
+// *skip != 0, so a captured solver graph can keep replaying after convergence.
+template <typename T>
+int dot2_async(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev_out4, bool comp, const int* skip);
+template <typename T>
+int nrm2_async(hpb_handle_t h, size_t n, const T* x, T* dev_out4, bool comp, const int* skip);
+template <typename T>
+int multidot_async(hpb_handle_t h, int k, const T* const* V, size_t n, const T* w, bool comp, T* dev_out4k,
+                   const int* skip);
+template <typename T>
+int multiaxpy_devcoef(hpb_handle_t h, int k, const T* dev_a, const T* const* V, size_t n, T* y, bool comp,
+                      const int* skip);
+// Device-resident GMRES (CGS2) / BiCGStab(L <= 8): vectors T, scalars FP64.
+// cache (optional) keeps the solver's buffers and captured graph between calls
+// with the same operands; release it with dsolve_release.
+struct dsolver_base {
+    virtual ~dsolver_base() {}
+};
+template <typename T>
+int dsolve(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, const T* val, const T* b, T* x,
+           const hpb_solver_params& p, hpb_solver_result* res, double* history, int history_cap,
+           dsolver_base** cache = nullptr);
+void dsolve_release(dsolver_base* cache);
+// b == null: y = A x; else y = b - A x
+template <typename T>
+int csr_apply(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* col, const T* val, const T* x,
+              const T* b, T* y, bool comp, const int* skip);
+
 }  // namespace hpb
 
 #define HPB_HIP(call)                                                            \

@@ -48,8 +48,9 @@ __device__ __forceinline__ T ld_ag(const T* p)
 template <typename T, int KC, bool COMP, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T> V, const T* __restrict__ w,
                                                     T* __restrict__ part, unsigned* __restrict__ ticket,
-                                                    T* __restrict__ out)
+                                                    T* __restrict__ out, const int* __restrict__ skip)
 {
+    if (skip != nullptr && *skip) return;   // solver graphs: flag set, launch does nothing
     using Vt = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
     constexpr int W = VEC ? dev::vec16<T>::W : 1;
     const unsigned G = gridDim.x;
@@ -144,8 +145,9 @@ __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T>
 // y := y + sum_j a_j V_j (compensated: Dot2 over the k+1 terms of each element)
 template <typename T, bool COMP, bool VEC, bool DEVCOEF>
 __global__ __launch_bounds__(256) void k_multiaxpy(size_t n, int k, coef_pack<T> a, const T* __restrict__ da,
-                                                   ptr_pack<T> V, T* __restrict__ y)
+                                                   ptr_pack<T> V, T* __restrict__ y, const int* __restrict__ skip)
 {
+    if (skip != nullptr && *skip) return;
     using Vt = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
     constexpr int W = VEC ? dev::vec16<T>::W : 1;
     const size_t nv = n / W;
@@ -200,7 +202,7 @@ static inline bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 template <typename T>
 static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, const T* w, int comp, T* dev_out,
-                          T* host_out)
+                          T* host_out, const int* skip = nullptr)
 {
     HPB_CLEAR_STALE();
     HPB_REQUIRE(h && V && (n == 0 || w), HPB_ERR_INVALID, "hpb_multidot: null argument");
@@ -227,11 +229,11 @@ static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, co
     unsigned* tk = h->ticket + 1;
     dim3 grid((unsigned)g, (unsigned)nch);
     if (comp) {
-        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, true, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
-        else hipLaunchKernelGGL((k_multidot<T, KC, true, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
+        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, true, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
+        else hipLaunchKernelGGL((k_multidot<T, KC, true, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
     } else {
-        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, false, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
-        else hipLaunchKernelGGL((k_multidot<T, KC, false, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out);
+        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, false, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
+        else hipLaunchKernelGGL((k_multidot<T, KC, false, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
     }
     HPB_LAUNCH_CHECK();
     if (!dev_out) {
@@ -245,7 +247,7 @@ static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, co
 
 template <typename T>
 static int multiaxpy_entry(hpb_handle_t h, int k, const T* a, const T* dev_a, const T* const* V, size_t n, T* y,
-                           int comp)
+                           int comp, const int* skip = nullptr)
 {
     HPB_CLEAR_STALE();
     HPB_REQUIRE(h && V && (a || dev_a) && (n == 0 || y), HPB_ERR_INVALID, "hpb_multiaxpy: null argument");
@@ -269,7 +271,7 @@ static int multiaxpy_entry(hpb_handle_t h, int k, const T* a, const T* dev_a, co
     if (g < 1) g = 1;
     const bool dc = dev_a != nullptr;
 #define HPB_MAXPY(C, VV, D) \
-    hipLaunchKernelGGL((k_multiaxpy<T, C, VV, D>), dim3((unsigned)g), dim3(256), 0, h->stream, n, k, ca, dev_a, pk, y)
+    hipLaunchKernelGGL((k_multiaxpy<T, C, VV, D>), dim3((unsigned)g), dim3(256), 0, h->stream, n, k, ca, dev_a, pk, y, skip)
     if (comp) {
         if (vec) { if (dc) HPB_MAXPY(true, true, true); else HPB_MAXPY(true, true, false); }
         else { if (dc) HPB_MAXPY(true, false, true); else HPB_MAXPY(true, false, false); }
@@ -281,6 +283,27 @@ static int multiaxpy_entry(hpb_handle_t h, int k, const T* a, const T* dev_a, co
     HPB_LAUNCH_CHECK();
     return HPB_OK;
 }
+
+template <typename T>
+int multidot_async(hpb_handle_t h, int k, const T* const* V, size_t n, const T* w, bool comp, T* dev_out4k,
+                   const int* skip)
+{
+    return multidot_entry<T>(h, k, V, n, w, comp ? 1 : 0, dev_out4k, nullptr, skip);
+}
+template <typename T>
+int multiaxpy_devcoef(hpb_handle_t h, int k, const T* dev_a, const T* const* V, size_t n, T* y, bool comp,
+                      const int* skip)
+{
+    return multiaxpy_entry<T>(h, k, nullptr, dev_a, V, n, y, comp ? 1 : 0, skip);
+}
+template int multidot_async<double>(hpb_handle_t, int, const double* const*, size_t, const double*, bool, double*,
+                                    const int*);
+template int multidot_async<float>(hpb_handle_t, int, const float* const*, size_t, const float*, bool, float*,
+                                   const int*);
+template int multiaxpy_devcoef<double>(hpb_handle_t, int, const double*, const double* const*, size_t, double*, bool,
+                                       const int*);
+template int multiaxpy_devcoef<float>(hpb_handle_t, int, const float*, const float* const*, size_t, float*, bool,
+                                      const int*);
 
 }  // namespace hpb
 

@@ -120,9 +120,12 @@ __device__ __forceinline__ bool take_ticket(unsigned* ticket, unsigned G)
 template <typename T, int OP, int BLOCK, int UNROLL, bool VEC, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict__ x, const T* __restrict__ y,
                                                   T* __restrict__ part, unsigned* __restrict__ ticket,
-                                                  T* __restrict__ out)
+                                                  T* __restrict__ out, const int* __restrict__ skip)
 {
     using tr = red_traits<OP>;
+    // solver graphs pass a device flag: once it is set the launch does nothing
+    // (every block returns, so the ticket is untouched)
+    if (skip != nullptr && *skip) return;
     using V = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
     constexpr int W = VEC ? dev::vec16<T>::W : 1;
     constexpr size_t TILE = (size_t)BLOCK * UNROLL;
@@ -228,32 +231,32 @@ static red_shape reduce_shape(const hpb_handle_s* h, size_t n, bool aligned, boo
 
 template <typename T, int OP, int BLOCK, int UNROLL>
 static void launch_bu(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
-                      unsigned* ticket, T* out)
+                      unsigned* ticket, T* out, const int* skip)
 {
     if (sh.vecw > 1 && (sh.flags & HPB_TUNE_NONTEMPORAL) && UNROLL >= 2)
         hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true, true>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y,
-                           part, ticket, out);
+                           part, ticket, out, skip);
     else if (sh.vecw > 1)
         hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y, part,
-                           ticket, out);
+                           ticket, out, skip);
     else
         hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, false>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y, part,
-                           ticket, out);
+                           ticket, out, skip);
 }
 
 template <typename T, int OP>
 static void launch_red(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
-                       unsigned* ticket, T* out)
+                       unsigned* ticket, T* out, const int* skip)
 {
     switch (sh.block * 16 + sh.unroll) {
-    case 256 * 16 + 1: launch_bu<T, OP, 256, 1>(sh, st, n, x, y, part, ticket, out); break;
-    case 256 * 16 + 2: launch_bu<T, OP, 256, 2>(sh, st, n, x, y, part, ticket, out); break;
-    case 256 * 16 + 8: launch_bu<T, OP, 256, 8>(sh, st, n, x, y, part, ticket, out); break;
-    case 512 * 16 + 1: launch_bu<T, OP, 512, 1>(sh, st, n, x, y, part, ticket, out); break;
-    case 512 * 16 + 2: launch_bu<T, OP, 512, 2>(sh, st, n, x, y, part, ticket, out); break;
-    case 512 * 16 + 4: launch_bu<T, OP, 512, 4>(sh, st, n, x, y, part, ticket, out); break;
-    case 512 * 16 + 8: launch_bu<T, OP, 512, 8>(sh, st, n, x, y, part, ticket, out); break;
-    default: launch_bu<T, OP, 256, 4>(sh, st, n, x, y, part, ticket, out); break;
+    case 256 * 16 + 1: launch_bu<T, OP, 256, 1>(sh, st, n, x, y, part, ticket, out, skip); break;
+    case 256 * 16 + 2: launch_bu<T, OP, 256, 2>(sh, st, n, x, y, part, ticket, out, skip); break;
+    case 256 * 16 + 8: launch_bu<T, OP, 256, 8>(sh, st, n, x, y, part, ticket, out, skip); break;
+    case 512 * 16 + 1: launch_bu<T, OP, 512, 1>(sh, st, n, x, y, part, ticket, out, skip); break;
+    case 512 * 16 + 2: launch_bu<T, OP, 512, 2>(sh, st, n, x, y, part, ticket, out, skip); break;
+    case 512 * 16 + 4: launch_bu<T, OP, 512, 4>(sh, st, n, x, y, part, ticket, out, skip); break;
+    case 512 * 16 + 8: launch_bu<T, OP, 512, 8>(sh, st, n, x, y, part, ticket, out, skip); break;
+    default: launch_bu<T, OP, 256, 4>(sh, st, n, x, y, part, ticket, out, skip); break;
     }
 }
 
@@ -262,7 +265,8 @@ static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; 
 // dev_out != nullptr: asynchronous, result left in device memory.
 // otherwise: result copied to host_out (4 values), stream synchronised.
 template <typename T, int OP>
-static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev_out, T* host_out)
+static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev_out, T* host_out,
+                        const int* skip = nullptr)
 {
     HPB_CLEAR_STALE();
     HPB_REQUIRE(h, HPB_ERR_INVALID, "reduction: null handle");
@@ -272,7 +276,7 @@ static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev
     const bool al = aligned16(x) && (!red_traits<OP>::binary || aligned16(y));
     red_shape sh = reduce_shape<T>(h, n, al, red_traits<OP>::binary);
     T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
-    launch_red<T, OP>(sh, h->stream, n, x, y, reinterpret_cast<T*>(h->part), h->ticket, out);
+    launch_red<T, OP>(sh, h->stream, n, x, y, reinterpret_cast<T*>(h->part), h->ticket, out, skip);
     HPB_LAUNCH_CHECK();
     if (!dev_out) {
         HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T), hipMemcpyDeviceToHost, h->stream));
@@ -282,6 +286,24 @@ static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev
     }
     return HPB_OK;
 }
+
+// Internal asynchronous launchers for the solver graphs (see hpb_internal.hpp).
+template <typename T>
+int dot2_async(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev_out4, bool comp, const int* skip)
+{
+    return comp ? reduce_entry<T, R_DOT2>(h, n, x, y, dev_out4, nullptr, skip)
+                : reduce_entry<T, R_DOT>(h, n, x, y, dev_out4, nullptr, skip);
+}
+template <typename T>
+int nrm2_async(hpb_handle_t h, size_t n, const T* x, T* dev_out4, bool comp, const int* skip)
+{
+    return comp ? reduce_entry<T, R_NRM2C>(h, n, x, nullptr, dev_out4, nullptr, skip)
+                : reduce_entry<T, R_NRM2>(h, n, x, nullptr, dev_out4, nullptr, skip);
+}
+template int dot2_async<double>(hpb_handle_t, size_t, const double*, const double*, double*, bool, const int*);
+template int dot2_async<float>(hpb_handle_t, size_t, const float*, const float*, float*, bool, const int*);
+template int nrm2_async<double>(hpb_handle_t, size_t, const double*, double*, bool, const int*);
+template int nrm2_async<float>(hpb_handle_t, size_t, const float*, float*, bool, const int*);
 
 }  // namespace hpb
 

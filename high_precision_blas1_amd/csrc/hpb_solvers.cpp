@@ -99,6 +99,14 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
     return ok;
 }
 
+// the device-resident path covers CGS2 GMRES and BiCGStab(L <= 8)
+bool use_device(const hpb_solver_params& p)
+{
+    if (!p.device_resident) return false;
+    if (p.solver == HPB_SOLVER_GMRES) return p.gram_schmidt == 1 && !p.reorthogonalization;
+    return p.basis <= 8;
+}
+
 int validate(const hpb_solver_params* p)
 {
     if (!p) return HPB_ERR_INVALID;
@@ -131,6 +139,7 @@ void hpb_solver_params_default(hpb_solver_params* p)
     p->resid_recalc_freq = 1;
     p->save_history = 0;
     p->verbose = 0;
+    p->device_resident = 1;
 }
 
 int hpb_csr_solve_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
@@ -152,7 +161,14 @@ int hpb_csr_solve_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr,
         st::op_t op(&vo, &A);
         HPB_HIP(hipStreamSynchronize(h->stream));
         const double t0 = now_s();
-        const bool ok = run_solver<double>(*p, &log, vo, op, b, x, res, history, history_cap, p->rel_tol);
+        bool ok;
+        if (use_device(*p)) {
+            const int rc = hpb::dsolve<double>(h, rows, nnz, row_ptr, col_ind, val, b, x, *p, res, history, history_cap);
+            if (rc != HPB_OK) return rc;
+            ok = res->converged != 0;
+        } else {
+            ok = run_solver<double>(*p, &log, vo, op, b, x, res, history, history_cap, p->rel_tol);
+        }
         HPB_HIP(hipStreamSynchronize(h->stream));
         res->seconds = now_s() - t0;
         double* work = nullptr;
@@ -187,6 +203,9 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
         if (inner->high_precision) vo32.use_high_precision();
         s64::mat_t A64(h);
         A64.adopt_device(rows, rows, nnz, (int*)row_ptr, (int*)col_ind, (double*)val, false);
+        // the timed region includes building the FP32 copy of the matrix values
+        HPB_HIP(hipStreamSynchronize(h->stream));
+        const double t0 = now_s();
         float* val32 = nullptr;
         hpb::check(hpb_malloc((void**)&val32, sizeof(float) * (size_t)nnz), "ir: matrix values");
         struct dev_guard {
@@ -204,8 +223,23 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
         vo64.start_use_vector(r);
         vo32.start_use_vector(r32);
         vo32.start_use_vector(d32);
-        HPB_HIP(hipStreamSynchronize(h->stream));
-        const double t0 = now_s();
+        struct vec_guard {
+            s64::vo_t& v64;
+            s32::vo_t& v32;
+            double*& r;
+            float*& r32;
+            float*& d32;
+            ~vec_guard()
+            {
+                v64.free_vector(r);
+                v32.free_vector(r32);
+                v32.free_vector(d32);
+            }
+        } vecs{vo64, vo32, r, r32, d32};
+        struct cache_guard {
+            hpb::dsolver_base* c = nullptr;
+            ~cache_guard() { hpb::dsolve_release(c); }
+        } inner_cache;
         double ob[4];
         hpb::check(hpb_nrm2c_f64(h, n, b, ob), "ir: rhs norm");
         const double bnorm = ob[3];
@@ -228,7 +262,13 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
             hpb::check(hpb_scale_convert_f64_to_f32(h, n, 1.0 / rn, r, r32), "ir: scale residual");
             vo32.assign_scalar(0.0f, d32);
             hpb_solver_result ir{};
-            run_solver<float>(*inner, &log, vo32, op32, r32, d32, &ir, nullptr, 0, inner->rel_tol);
+            if (use_device(*inner)) {
+                const int rc = hpb::dsolve<float>(h, rows, nnz, row_ptr, col_ind, val32, r32, d32, *inner, &ir,
+                                                  nullptr, 0, &inner_cache.c);
+                if (rc != HPB_OK) return rc;
+            } else {
+                run_solver<float>(*inner, &log, vo32, op32, r32, d32, &ir, nullptr, 0, inner->rel_tol);
+            }
             inner_total += ir.iters;
             hpb::check(hpb_axpy_f32_to_f64(h, n, rn, d32, x), "ir: update");
         }
@@ -240,9 +280,6 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
         res->resid_norm = rn;
         res->rhs_norm = bnorm;
         res->true_rel_resid = true_rel_residual(h, A64, b, x, r);
-        vo64.free_vector(r);
-        vo32.free_vector(r32);
-        vo32.free_vector(d32);
     } catch (const std::exception& e) {
         hpb::set_error(__FILE__, __LINE__, e.what());
         return HPB_ERR_INVALID;

@@ -20,7 +20,8 @@ class SolverParams(ctypes.Structure):
                 ("max_iters", ctypes.c_int), ("min_iters", ctypes.c_int), ("rel_tol", ctypes.c_double),
                 ("abs_tol", ctypes.c_double), ("high_precision", ctypes.c_int), ("compensated_spmv", ctypes.c_int),
                 ("gram_schmidt", ctypes.c_int), ("early_exit", ctypes.c_int), ("reorthogonalization", ctypes.c_int),
-                ("resid_recalc_freq", ctypes.c_int), ("save_history", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("resid_recalc_freq", ctypes.c_int), ("save_history", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("device_resident", ctypes.c_int)]
 
 
 class SolverResult(ctypes.Structure):

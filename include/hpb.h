@@ -310,6 +310,10 @@ typedef struct hpb_solver_params {
     int resid_recalc_freq;   /* BiCGStab(L): recompute b - Ax every k cycles (0 never) */
     int save_history;        /* keep (iteration, ||r||/||b||) pairs */
     int verbose;             /* 0 silent, 1 errors, 2 per-check info */
+    int device_resident;     /* 1: iterations run on the device (scalars in device memory, one captured
+                                hipGraph per GMRES restart cycle / BiCGStab(L) cycle, no host round trip
+                                per dot); applies to CGS2 GMRES and BiCGStab(L <= 8), other settings run
+                                the host-driven templates.  Same recurrences and monitor rules. */
 } hpb_solver_params;
 typedef struct hpb_solver_result {
     int converged;           /* monitor converged (true residual for GMRES restarts) */

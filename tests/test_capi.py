@@ -56,3 +56,23 @@ def test_header_is_c_compatible(tmp_path):
     src.write_text('#include "hpb.h"\nint main(void){return 0;}\n')
     subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                            str(src), "-o", str(tmp_path / "t")])
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """The ctypes mirrors of hpb_solver_params / hpb_solver_result have the C layout."""
+    from high_precision_blas1_amd.solvers import SolverParams, SolverResult
+    lines = []
+    for cname, py in (("hpb_solver_params", SolverParams), ("hpb_solver_result", SolverResult)):
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    src = tmp_path / "l.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "hpb.h"\nint main(void){\n%s\nreturn 0;}\n'
+                   % "\n".join(lines))
+    subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                           "-o", str(tmp_path / "l")])
+    got = dict(l.split() for l in subprocess.check_output([str(tmp_path / "l")], text=True).splitlines())
+    for cname, py in (("hpb_solver_params", SolverParams), ("hpb_solver_result", SolverResult)):
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)

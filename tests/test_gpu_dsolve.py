@@ -1,0 +1,113 @@
+"""Device-resident GMRES(m) / BiCGStab(L) (device scalars, captured hipGraphs)
+against the host-driven solver templates on the same problems, and the monitor
+rules (tolerance, iteration cap, minimum iterations, invalid input, history)."""
+import numpy as np
+import pytest
+
+import high_precision_blas1_amd as H
+import high_precision_blas1_amd.solvers as S
+import oracle as O
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(handle, nx=14, coef=S.CONVDIFF7, seed=4):
+    Mh = O.stencil7(nx, nx, nx, coef)
+    A = S.DeviceCSR.stencil7(handle, nx, nx, nx, coef)
+    b = Mh @ np.random.default_rng(seed).uniform(-1, 1, Mh.shape[0])
+    return Mh, A, b
+
+
+def _run(handle, A, b, hist=0, **kw):
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    res = S.solve(handle, A, db, dx, S.params(**kw), history_cap=hist)
+    x = dx.numpy()
+    db.free()
+    dx.free()
+    return res, x
+
+
+@pytest.mark.parametrize("m,high,ee", [(30, 1, 1), (30, 0, 1), (20, 1, 0), (7, 1, 1)])
+def test_gmres_device_matches_host(handle, m, high, ee):
+    Mh, A, b = _setup(handle)
+    say("dgmres", m, high, ee)
+    kw = dict(solver=S.GMRES, restart=m, rel_tol=1e-11, max_iters=4000, high_precision=high, gram_schmidt=1,
+              early_exit=ee)
+    rd, xd = _run(handle, A, b, device_resident=1, **kw)
+    rh, xh = _run(handle, A, b, device_resident=0, **kw)
+    assert rd["converged"] == 1 and rh["converged"] == 1
+    assert abs(rd["iters"] - rh["iters"]) <= 1, (rd["iters"], rh["iters"])
+    assert rd["true_rel_resid"] <= 1.5e-11
+    assert np.linalg.norm(xd - xh) <= 1e-8 * np.linalg.norm(xh)
+    A.free()
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 8])
+def test_bicgstabl_device_matches_host(handle, L):
+    Mh, A, b = _setup(handle)
+    say("dbicgstab", L)
+    kw = dict(solver=S.BICGSTABL, basis=L, rel_tol=1e-11, max_iters=4000, high_precision=1, resid_recalc_freq=1)
+    rd, xd = _run(handle, A, b, device_resident=1, **kw)
+    rh, xh = _run(handle, A, b, device_resident=0, **kw)
+    assert rd["converged"] == 1 and rd["breakdown"] == 0
+    assert rd["true_rel_resid"] <= 1.5e-11
+    # same recurrences; the MR part solves the same least-squares problem by
+    # normal equations instead of modified Gram-Schmidt
+    assert abs(rd["iters"] - rh["iters"]) <= max(2 * L, rh["iters"] // 10), (rd["iters"], rh["iters"])
+    assert np.linalg.norm(Mh @ xd - b) <= 1.5e-11 * np.linalg.norm(b)
+    A.free()
+
+
+def test_monitor_rules(handle):
+    Mh, A, b = _setup(handle, nx=10)
+    # iteration cap
+    r, _ = _run(handle, A, b, solver=S.GMRES, restart=5, rel_tol=1e-15, max_iters=12)
+    assert r["converged"] == 0 and r["iters"] == 12
+    r, _ = _run(handle, A, b, solver=S.BICGSTABL, basis=4, rel_tol=1e-15, max_iters=10)
+    assert r["converged"] == 0 and r["iters"] == 12
+    # minimum iterations
+    r, _ = _run(handle, A, b, solver=S.BICGSTABL, basis=2, rel_tol=1e-1, max_iters=100, min_iters=20)
+    assert r["converged"] == 1 and r["iters"] >= 20
+    r, _ = _run(handle, A, b, solver=S.GMRES, restart=10, rel_tol=1e-1, max_iters=100, min_iters=25)
+    assert r["converged"] == 1 and r["iters"] >= 25
+    # zero right-hand side: converged before the first step, x = 0
+    r, x = _run(handle, A, np.zeros_like(b), solver=S.GMRES, restart=10, rel_tol=1e-10)
+    assert r["converged"] == 1 and r["iters"] == 0 and not x.any()
+    r, x = _run(handle, A, np.zeros_like(b), solver=S.BICGSTABL, basis=2, rel_tol=1e-10)
+    assert r["converged"] == 1 and r["iters"] == 0 and not x.any()
+    # invalid input
+    bb = b.copy()
+    bb[5] = np.inf
+    for solver in (S.GMRES, S.BICGSTABL):
+        r, _ = _run(handle, A, bb, solver=solver, restart=10, basis=2, rel_tol=1e-10, max_iters=50)
+        assert r["converged"] == 0
+    A.free()
+
+
+def test_history(handle):
+    Mh, A, b = _setup(handle, nx=10)
+    for solver in (S.GMRES, S.BICGSTABL):
+        r, _ = _run(handle, A, b, hist=500, solver=solver, restart=10, basis=2, rel_tol=1e-10, max_iters=2000,
+                    save_history=1)
+        h = np.array(r["history"])
+        assert r["history_len"] == len(h) >= 2
+        assert h[0, 0] == 0 and abs(h[0, 1] - 1.0) < 1e-12
+        assert h[-1, 1] <= 1e-10 and h[-1, 0] == r["iters"]
+        assert np.all(np.diff(h[:, 0]) > 0)
+    A.free()
+
+
+def test_repeated_solves_reuse_nothing_stale(handle):
+    """Two solves in a row with different right-hand sides give the same answers
+    as fresh solves (the device state and graphs are per call)."""
+    Mh, A, b = _setup(handle, nx=10)
+    b2 = Mh @ np.random.default_rng(9).uniform(-1, 1, Mh.shape[0])
+    kw = dict(solver=S.BICGSTABL, basis=2, rel_tol=1e-12, max_iters=2000)
+    r1, x1 = _run(handle, A, b, **kw)
+    r2, x2 = _run(handle, A, b2, **kw)
+    r1b, x1b = _run(handle, A, b, **kw)
+    assert r1["iters"] == r1b["iters"] and np.array_equal(x1, x1b)
+    assert r2["converged"] == 1
+    A.free()

@@ -37,15 +37,16 @@ def _host_rel(Mh, b, x):
     return np.linalg.norm(b - Mh @ x) / np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("dev", [0, 1])
 @pytest.mark.parametrize("gs", [0, 1])
 @pytest.mark.parametrize("high", [0, 1])
-def test_gmres_small(handle, gs, high):
+def test_gmres_small(handle, gs, high, dev):
     Mh, A, b = _problem(handle, 16, 14, 12, S.CONVDIFF7)
-    say("gmres", gs, high)
+    say("gmres", gs, high, dev)
     db = H.DeviceArray.from_numpy(b, handle)
     dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
     p = S.params(solver=S.GMRES, restart=30, rel_tol=1e-12, max_iters=3000, high_precision=high, gram_schmidt=gs,
-                 early_exit=0)
+                 early_exit=0, device_resident=dev)
     res = S.solve(handle, A, db, dx, p)
     x = dx.numpy()
     assert res["converged"] == 1
@@ -58,13 +59,15 @@ def test_gmres_small(handle, gs, high):
     A.free()
 
 
+@pytest.mark.parametrize("dev", [0, 1])
 @pytest.mark.parametrize("L", [1, 2, 4])
-def test_bicgstabl_small(handle, L):
+def test_bicgstabl_small(handle, L, dev):
     Mh, A, b = _problem(handle, 16, 14, 12, S.CONVDIFF7)
-    say("bicgstabl", L)
+    say("bicgstabl", L, dev)
     db = H.DeviceArray.from_numpy(b, handle)
     dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
-    p = S.params(solver=S.BICGSTABL, basis=L, rel_tol=1e-12, max_iters=5000, high_precision=1, resid_recalc_freq=1)
+    p = S.params(solver=S.BICGSTABL, basis=L, rel_tol=1e-12, max_iters=5000, high_precision=1, resid_recalc_freq=1,
+                 device_resident=dev)
     res = S.solve(handle, A, db, dx, p)
     assert res["converged"] == 1 and res["breakdown"] == 0
     x = dx.numpy()
@@ -74,14 +77,15 @@ def test_bicgstabl_small(handle, L):
     A.free()
 
 
+@pytest.mark.parametrize("dev", [0, 1])
 @pytest.mark.parametrize("inner_solver,arg", [(S.GMRES, 30), (S.BICGSTABL, 2)])
-def test_mixed_precision_refinement(handle, inner_solver, arg):
+def test_mixed_precision_refinement(handle, inner_solver, arg, dev):
     Mh, A, b = _problem(handle, 24, 24, 24, S.CONVDIFF7)
-    say("ir", inner_solver, arg)
+    say("ir", inner_solver, arg, dev)
     db = H.DeviceArray.from_numpy(b, handle)
     dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
     inner = S.params(solver=inner_solver, restart=arg, basis=arg, rel_tol=1e-5, max_iters=2000, high_precision=1,
-                     early_exit=1, gram_schmidt=1)
+                     early_exit=1, gram_schmidt=1, device_resident=dev)
     res = S.solve_ir(handle, A, db, dx, inner, rel_tol=1e-12, max_outer=30)
     assert res["converged"] == 1
     assert res["true_rel_resid"] <= 1e-12
@@ -90,17 +94,21 @@ def test_mixed_precision_refinement(handle, inner_solver, arg):
     A.free()
 
 
-def test_failure_paths(handle):
+@pytest.mark.parametrize("dev", [0, 1])
+def test_failure_paths(handle, dev):
     Mh, A, b = _problem(handle, 10, 10, 10, S.CONVDIFF7)
+    say("failure paths", dev)
     db = H.DeviceArray.from_numpy(b, handle)
     dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
-    res = S.solve(handle, A, db, dx, S.params(solver=S.GMRES, restart=5, rel_tol=1e-14, max_iters=10))
+    res = S.solve(handle, A, db, dx, S.params(solver=S.GMRES, restart=5, rel_tol=1e-14, max_iters=10,
+                                             device_resident=dev))
     assert res["converged"] == 0 and res["iters"] >= 10
     bb = b.copy()
     bb[3] = np.nan
     db.upload(bb)
     dx.upload(np.zeros_like(b))
-    res = S.solve(handle, A, db, dx, S.params(solver=S.BICGSTABL, basis=2, rel_tol=1e-10, max_iters=100))
+    res = S.solve(handle, A, db, dx, S.params(solver=S.BICGSTABL, basis=2, rel_tol=1e-10, max_iters=100,
+                                             device_resident=dev))
     assert res["converged"] == 0
     with pytest.raises(H.HPBError):
         S.solve(handle, A, db, dx, S.params(solver=S.GMRES, restart=0))
@@ -117,7 +125,10 @@ def test_convdiff_128_gmres_and_bicgstab(handle):
     db = H.DeviceArray(n, np.float64, handle)
     A.spmv(xs, db, compensated=True)
     for p in (S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000, gram_schmidt=1, early_exit=1),
-              S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000)):
+              S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000),
+              S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000, gram_schmidt=1, early_exit=1,
+                       device_resident=0),
+              S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000, device_resident=0)):
         dx = H.DeviceArray(n, np.float64, handle)
         H.call("hpb_fill_f64", handle.ptr, n, ctypes.c_double(0.0), dx.ptr)
         res = S.solve(handle, A, db, dx, p)
