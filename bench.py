@@ -263,98 +263,178 @@ def cfg_spmv(H, h, log):
 
 
 def _solve_problem(H, h, S, nx, coef, seed):
+    """Matrix, b = A x* with x* uniform in [-1, 1] (Philox seed), and x*."""
     import numpy as np
     A = S.DeviceCSR.stencil7(h, nx, nx, nx, coef)
     xs = H.DeviceArray(A.rows, np.float64, h)
     H.call("hpb_random_uniform_f64", h.ptr, A.rows, seed, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), xs.ptr)
     b = H.DeviceArray(A.rows, np.float64, h)
     A.spmv(xs, b, compensated=True)
-    xs.free()
-    return A, b
+    return A, b, xs
 
 
-def _run_solve(H, h, S, A, b, p, ir=False):
+# ---- algorithmic bytes of the solvers (inputs read once, outputs written once;
+# the SpMV reads x once, as the stencil rows of neighbouring threads share it)
+def _spmv_bytes(n, nnz, sval, svec, residual=False):
+    return (4 + sval) * nnz + 4 * (n + 1) + 2 * svec * n + (svec * n if residual else 0)
+
+
+def _bicgstabl_bytes(L, n, nnz, sval, s, cycles):
+    S = _spmv_bytes(n, nnz, sval, s)
+    b = 0
+    for j in range(L):
+        b += 2 * n * s + 3 * (j + 1) * n * s + S          # (r_j, r~), u updates, u_{j+1} = A u_j
+        b += 2 * n * s + 3 * (j + 1) * n * s + 3 * n * s + S   # (u_{j+1}, r~), r and x updates, r_{j+1} = A r_j
+    b += sum((k + 2) * n * s for k in range(1, L + 1))    # Gram multi-dots
+    b += (2 * L + 6) * n * s                               # MR update of x, r_0, u_0
+    b += _spmv_bytes(n, nnz, sval, s, True) + n * s        # true residual and its norm
+    return cycles * b + 3 * n * s + _spmv_bytes(n, nnz, sval, s, True)
+
+
+def _gmres_bytes(m, n, nnz, sval, s, iters):
+    S = _spmv_bytes(n, nnz, sval, s)
+    def step(j):   # CGS2: two (multi-dot + multi-update) passes, norm, scale
+        return S + 2 * ((j + 2) * n * s + (j + 3) * n * s) + n * s + 2 * n * s
+    cycles, rest = divmod(iters, m)
+    restart = _spmv_bytes(n, nnz, sval, s, True) + 3 * n * s
+    b = cycles * (sum(step(j) for j in range(m)) + restart + (m + 2) * n * s)
+    if rest:
+        b += sum(step(j) for j in range(rest)) + restart + (rest + 2) * n * s
+    return b + restart
+
+
+def _solve_bytes(p, n, nnz, iters, sval=8, s=8):
+    if p.solver == 1:
+        return _bicgstabl_bytes(p.basis, n, nnz, sval, s, -(-iters // p.basis))
+    return _gmres_bytes(p.restart, n, nnz, sval, s, iters)
+
+
+def _run_solve(H, h, S, A, b, xs, p, ir=False):
     import numpy as np
     x = H.DeviceArray(A.rows, np.float64, h)
     H.call("hpb_fill_f64", h.ptr, A.rows, ctypes.c_double(0.0), x.ptr)
     r = S.solve_ir(h, A, b, x, p, rel_tol=1e-12, max_outer=40) if ir else S.solve(h, A, b, x, p)
+    # relative error against x*: ||x - x*|| / ||x*|| (compensated norms)
+    e = H.DeviceArray(A.rows, np.float64, h)
+    H.call("hpb_axpbyc_f64", h.ptr, A.rows, ctypes.c_double(1.0), x.ptr, ctypes.c_double(-1.0), xs.ptr, e.ptr)
+    o1, o2 = np.zeros(4), np.zeros(4)
+    H.call("hpb_nrm2c_f64", h.ptr, A.rows, e.ptr, o1.ctypes.data_as(ctypes.c_void_p))
+    H.call("hpb_nrm2c_f64", h.ptr, A.rows, xs.ptr, o2.ctypes.data_as(ctypes.c_void_p))
     x.free()
-    return {k: (round(v, 6) if isinstance(v, float) and k == "seconds" else v)
-            for k, v in r.items() if k in ("converged", "iters", "outer_iters", "true_rel_resid", "seconds")}
+    e.free()
+    out = {k: (round(v, 6) if isinstance(v, float) and k == "seconds" else v)
+           for k, v in r.items() if k in ("converged", "iters", "outer_iters", "true_rel_resid", "seconds")}
+    out["rel_err_vs_xstar"] = float(o1[3] / o2[3])
+    n, nnz = A.rows, A.nnz
+    if ir:
+        # inner FP32 solves (FP32 values, int32 indices, FP32 vectors) + per sweep:
+        # FP64 compensated residual, its norm, scaling to FP32, FP64 update; plus the
+        # FP32 copy of the values
+        inner = _solve_bytes(p, n, nnz, r["iters"], sval=4, s=4)
+        sweep = _spmv_bytes(n, nnz, 8, 8, True) + 8 * n + 12 * n + 20 * n
+        out["bytes_moved"] = int(inner + (r["outer_iters"] + 1) * sweep + 12 * nnz)
+    else:
+        out["bytes_moved"] = int(_solve_bytes(p, n, nnz, r["iters"]))
+    return out
 
 
 def cfg_krylov(H, h, log):
     import high_precision_blas1_amd.solvers as S
     log("krylov 128^3 convection-diffusion")
-    A, b = _solve_problem(H, h, S, 128, S.CONVDIFF7, 2)
+    A, b, xs = _solve_problem(H, h, S, 128, S.CONVDIFF7, 2)
     res = {"matrix": "128^3 first-order upwind convection-diffusion, v=(1,0.5,0.25), cell Peclet 100",
            "rows": A.rows, "nnz": A.nnz}
+    G = dict(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000)
+    B = dict(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000)
     for name, p in (
-            ("gmres50_naive", S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000, high_precision=0)),
-            ("gmres50_compensated", S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000,
-                                             high_precision=1)),
-            ("gmres50_compensated_mgs_reference_schedule",
-             S.params(solver=S.GMRES, restart=50, rel_tol=1e-12, max_iters=20000, high_precision=1, gram_schmidt=0,
-                      early_exit=0)),
-            ("bicgstab4_naive", S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000,
-                                         high_precision=0)),
-            ("bicgstab4_compensated", S.params(solver=S.BICGSTABL, basis=4, rel_tol=1e-12, max_iters=20000,
-                                               high_precision=1))):
+            ("gmres50_naive", S.params(high_precision=0, **G)),
+            ("gmres50_compensated", S.params(high_precision=1, **G)),
+            ("gmres50_compensated_host_driven", S.params(high_precision=1, device_resident=0, **G)),
+            ("gmres50_compensated_mgs_reference_schedule", S.params(high_precision=1, gram_schmidt=0, early_exit=0,
+                                                                     **G)),
+            ("bicgstab4_naive", S.params(high_precision=0, **B)),
+            ("bicgstab4_compensated", S.params(high_precision=1, **B)),
+            ("bicgstab4_compensated_host_driven", S.params(high_precision=1, device_resident=0, **B))):
         log(name)
-        res[name] = _run_solve(H, h, S, A, b, p)
-    A.free()
-    b.free()
+        res[name] = _run_solve(H, h, S, A, b, xs, p)
+    for a in (A, b, xs):
+        a.free()
+    return res
+
+
+def _mixed_on(H, h, S, log, nx, coef, seed, plain, mixed):
+    A, b, xs = _solve_problem(H, h, S, nx, coef, seed)
+    res = {"rows": A.rows, "nnz": A.nnz}
+    for k, p in plain.items():
+        log(k)
+        res[k] = _run_solve(H, h, S, A, b, xs, p)
+    for k, p in mixed.items():
+        log(k)
+        res[k] = _run_solve(H, h, S, A, b, xs, p, ir=True)
+    for a, m in zip(plain, mixed):
+        if res[a]["converged"] and res[m]["converged"]:
+            res["speedup_" + m] = round(res[a]["seconds"] / res[m]["seconds"], 3)
+            res["bytes_ratio_" + m] = round(res[a]["bytes_moved"] / res[m]["bytes_moved"], 3)
+    for a in (A, b, xs):
+        a.free()
     return res
 
 
 def cfg_mixed(H, h, log):
+    """FP32-storage inner GMRES(30) / BiCGStab(2) with compensated FP32 dots inside FP64
+    refinement vs the plain FP64 compensated solvers, time to 1e-12 true residual."""
     import high_precision_blas1_amd.solvers as S
+    res = {}
     log("mixed precision 128^3 convection-diffusion")
-    A, b = _solve_problem(H, h, S, 128, S.CONVDIFF7, 2)
-    res = {"matrix": "128^3 convection-diffusion (as krylov)"}
-    plain = {
-        "gmres_fp64": S.params(solver=S.GMRES, restart=30, rel_tol=1e-12, max_iters=20000, high_precision=1),
-        "bicgstab2_fp64": S.params(solver=S.BICGSTABL, basis=2, rel_tol=1e-12, max_iters=20000, high_precision=1)}
-    mixed = {
-        "gmres30_fp32_refined": S.params(solver=S.GMRES, restart=30, rel_tol=1e-5, max_iters=5000, high_precision=1),
-        "bicgstab2_fp32_refined": S.params(solver=S.BICGSTABL, basis=2, rel_tol=1e-5, max_iters=5000,
-                                           high_precision=1)}
-    for k, p in plain.items():
-        log(k)
-        res[k] = _run_solve(H, h, S, A, b, p)
-    for k, p in mixed.items():
-        log(k)
-        res[k] = _run_solve(H, h, S, A, b, p, ir=True)
-    for a, m in (("gmres_fp64", "gmres30_fp32_refined"), ("bicgstab2_fp64", "bicgstab2_fp32_refined")):
-        if res[a]["converged"] and res[m]["converged"]:
-            res["speedup_" + m] = round(res[a]["seconds"] / res[m]["seconds"], 3)
-    A.free()
-    b.free()
+    G = dict(solver=S.GMRES, restart=30, high_precision=1)
+    B = dict(solver=S.BICGSTABL, basis=2, high_precision=1)
+    res["convdiff128"] = _mixed_on(
+        H, h, S, log, 128, S.CONVDIFF7, 2,
+        {"gmres30_fp64": S.params(rel_tol=1e-12, max_iters=20000, **G),
+         "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
+        {"gmres30_fp32_refined": S.params(rel_tol=1e-6, max_iters=5000, **G),
+         "bicgstab2_fp32_refined": S.params(rel_tol=1e-6, max_iters=5000, **B)})
+    res["convdiff128"]["matrix"] = "128^3 convection-diffusion (as krylov)"
+    log("mixed precision 256^3 anisotropic diffusion")
+    res["aniso256"] = _mixed_on(
+        H, h, S, log, 256, S.ANISO7, 2,
+        {"bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
+        {"bicgstab2_fp32_refined": S.params(rel_tol=1e-4, max_iters=3000, **B)})
+    res["aniso256"]["matrix"] = "256^3 7-point anisotropic diffusion, k_z / k_x = 1e4, Dirichlet"
     return res
 
 
 def cfg_multidot(H, h, log):
-    """Fused compensated multi-dot, k = 1..64 at n = 2^24 (w read once per 8 columns)."""
+    """Fused compensated multi-dot: k = 1..64 at n = 2^24, and lengths 2^10..2^24 for
+    k in {1, 8, 32, 64} (w read once per chunk of 8 columns)."""
     import numpy as np
     log("multidot")
-    n = 1 << 24
+    nmax = 1 << 24
     kmax = 64
-    V = [H.DeviceArray(n, np.float64, h) for _ in range(kmax)]
-    w = H.DeviceArray(n, np.float64, h)
+    V = [H.DeviceArray(nmax, np.float64, h) for _ in range(kmax)]
+    w = H.DeviceArray(nmax, np.float64, h)
     for i, a in enumerate(V + [w]):
-        H.call("hpb_random_uniform_f64", h.ptr, n, 5, i, ctypes.c_double(-1.0), ctypes.c_double(1.0), a.ptr)
+        H.call("hpb_random_uniform_f64", h.ptr, nmax, 5, i, ctypes.c_double(-1.0), ctypes.c_double(1.0), a.ptr)
     d = H.DeviceArray(4 * kmax, np.float64, h)
     L = H.lib()
     rows = []
     for k in (1, 2, 4, 8, 16, 32, 50, 64):
         ptrs = (ctypes.c_void_p * k)(*[a.addr for a in V[:k]])
-        ms = timed(h, L.hpb_multidot_f64_async, (h.ptr, k, ptrs, n, w.ptr, 1, d.ptr), 10, 2)
-        nbytes = 8.0 * n * (k + 1)
+        ms = timed(h, L.hpb_multidot_f64_async, (h.ptr, k, ptrs, nmax, w.ptr, 1, d.ptr), 10, 2)
+        nbytes = 8.0 * nmax * (k + 1)
         rows.append({"k": k, "ms": round(ms, 4), "gbs_algorithmic": round(nbytes / ms / 1e6, 1),
                      "separate_dot2_bytes_ratio": round(2.0 * k / (k + 1), 3)})
+    sweep = []
+    for e in (10, 13, 16, 19, 22, 24):
+        n = 1 << e
+        for k in (1, 8, 32, 64):
+            ptrs = (ctypes.c_void_p * k)(*[a.addr for a in V[:k]])
+            ms = timed(h, L.hpb_multidot_f64_async, (h.ptr, k, ptrs, n, w.ptr, 1, d.ptr), 20, 3)
+            sweep.append({"n": n, "k": k, "us": round(1e3 * ms, 2),
+                          "gbs_algorithmic": round(8.0 * n * (k + 1) / ms / 1e6, 1)})
     for a in V + [w, d]:
         a.free()
-    return {"n": n, "rows": rows}
+    return {"n": nmax, "rows": rows, "length_sweep": sweep}
 
 
 def run_suite(H, h, x, y, n, args, log, no_cpu=False):

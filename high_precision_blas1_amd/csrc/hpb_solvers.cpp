@@ -22,6 +22,7 @@
 #include "csr/linear_operator.h"
 #include "hip_vector_operations.h"
 #include "hpb_internal.hpp"
+#include "lin_solvers/bicgstab.h"
 #include "lin_solvers/bicgstabl.h"
 #include "lin_solvers/default_monitor.h"
 #include "lin_solvers/gmres.h"
@@ -39,6 +40,7 @@ struct solver_types {
     using mon_t = default_monitor<vo_t, hpb::log_std>;
     using gmres_t = gmres<op_t, prec_t, vo_t, mon_t, hpb::log_std, 256>;
     using bicg_t = bicgstabl<op_t, prec_t, vo_t, mon_t, hpb::log_std>;
+    using bicg1_t = bicgstab<op_t, prec_t, vo_t, mon_t, hpb::log_std>;
 };
 
 double now_s()
@@ -87,6 +89,13 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
         ok = s.solve(op, (T*)b, x);
         fill(s.monitor());
         res->breakdown = 0;
+    } else if (p.solver == HPB_SOLVER_BICGSTAB) {
+        typename st::bicg1_t s(&vo, log, 1);
+        s.set_resid_recalc_freq(p.resid_recalc_freq);
+        s.monitor().init((T)rel_tol, (T)p.abs_tol, p.max_iters, p.min_iters, false, p.save_history != 0, true);
+        ok = s.solve(op, (T*)b, x);
+        fill(s.monitor());
+        res->breakdown = s.breakdown() ? 1 : 0;
     } else {
         typename st::bicg_t s(&vo, log, 1);
         s.set_basis_size(p.basis);
@@ -104,13 +113,15 @@ bool use_device(const hpb_solver_params& p)
 {
     if (!p.device_resident) return false;
     if (p.solver == HPB_SOLVER_GMRES) return p.gram_schmidt == 1 && !p.reorthogonalization;
+    if (p.solver == HPB_SOLVER_BICGSTAB) return false;
     return p.basis <= 8;
 }
 
 int validate(const hpb_solver_params* p)
 {
     if (!p) return HPB_ERR_INVALID;
-    if (p->solver != HPB_SOLVER_GMRES && p->solver != HPB_SOLVER_BICGSTABL) return HPB_ERR_INVALID;
+    if (p->solver != HPB_SOLVER_GMRES && p->solver != HPB_SOLVER_BICGSTABL && p->solver != HPB_SOLVER_BICGSTAB)
+        return HPB_ERR_INVALID;
     if (p->solver == HPB_SOLVER_GMRES && (p->restart < 1 || p->restart >= 256)) return HPB_ERR_INVALID;
     if (p->solver == HPB_SOLVER_BICGSTABL && (p->basis < 1 || p->basis > 30)) return HPB_ERR_INVALID;
     if (p->max_iters < 1 || !(p->rel_tol >= 0)) return HPB_ERR_INVALID;

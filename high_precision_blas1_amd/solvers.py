@@ -13,6 +13,7 @@ from . import DeviceArray, call
 
 GMRES = 0
 BICGSTABL = 1
+BICGSTAB = 2
 
 
 class SolverParams(ctypes.Structure):

@@ -296,8 +296,9 @@ int hpb_csr_diffusion2d_f64(hpb_handle_t h, int m, const double* kappa, int* row
  * hip_vector_operations and csr::hip_matrix, unpreconditioned, x0 = x on entry. */
 #define HPB_SOLVER_GMRES 0
 #define HPB_SOLVER_BICGSTABL 1
+#define HPB_SOLVER_BICGSTAB 2 /* plain BiCGStab, numerical_algos/lin_solvers/bicgstab.h:134-253 */
 typedef struct hpb_solver_params {
-    int solver;              /* HPB_SOLVER_GMRES or HPB_SOLVER_BICGSTABL */
+    int solver;              /* HPB_SOLVER_GMRES, HPB_SOLVER_BICGSTABL or HPB_SOLVER_BICGSTAB */
     int restart;             /* GMRES(m): m, 1 <= m < 256 (gmres.h:355-371) */
     int basis;               /* BiCGStab(L): L, 1..30 (bicgstabl.h:142-147) */
     int max_iters, min_iters;
@@ -307,7 +308,7 @@ typedef struct hpb_solver_params {
     int gram_schmidt;        /* GMRES: 0 MGS (the reference's), 1 CGS2 with fused kernels */
     int early_exit;          /* GMRES: leave a cycle when the Givens estimate meets the tolerance */
     int reorthogonalization; /* GMRES MGS: one extra correction per basis vector */
-    int resid_recalc_freq;   /* BiCGStab(L): recompute b - Ax every k cycles (0 never) */
+    int resid_recalc_freq;   /* BiCGStab(L) / BiCGStab: recompute b - Ax every k cycles / steps (0 never) */
     int save_history;        /* keep (iteration, ||r||/||b||) pairs */
     int verbose;             /* 0 silent, 1 errors, 2 per-check info */
     int device_resident;     /* 1: iterations run on the device (scalars in device memory, one captured

@@ -5,6 +5,8 @@ convergence histories of the GPU solvers can be compared:
   gmres_mgs ... source/numerical_algos/lin_solvers/gmres.h:376-494 (restarted,
                 modified Gram-Schmidt, Givens rotations, check at restarts)
   bicgstabl ... source/numerical_algos/lin_solvers/bicgstabl.h:151-274
+  bicgstab .... source/numerical_algos/lin_solvers/bicgstab.h:134-253 (no
+                preconditioner; residual recomputed every recalc steps)
 Dots are plain numpy (np.dot, pairwise order); these are references for the
 iteration behaviour, not for bits.  Pinned in tests/test_oracle_krylov.py against
 scipy's direct solver on small nonsymmetric problems.
@@ -129,4 +131,39 @@ def bicgstabl(A, b, x0, L, rel_tol, max_iters, recalc=1):
         if recalc and ((it // L + 1) % recalc == 0):
             r[0] = b - A @ x
         it += L
+    return x, it, hist
+
+
+def bicgstab(A, b, x0, rel_tol, max_iters, recalc=0):
+    """Plain BiCGStab; returns x, iterations, [(iteration, ||r||/||b||)]."""
+    x = x0.copy()
+    bn = np.linalg.norm(b)
+    tol = rel_tol * bn
+    r = b - A @ x
+    rs = r.copy()
+    rho_prev = alpha = omega_prev = 1.0
+    nu = np.zeros_like(b)
+    p = np.zeros_like(b)
+    it = 0
+    hist = []
+    while True:
+        rn = np.linalg.norm(r)
+        hist.append((it, rn / bn))
+        if rn <= tol or it >= max_iters:
+            break
+        rho = np.dot(r, rs)
+        beta = (rho / rho_prev) * (alpha / omega_prev)
+        p = r - beta * omega_prev * nu + beta * p
+        nu = A @ p
+        alpha = rho / np.dot(nu, rs)
+        s = r - alpha * nu
+        t = A @ s
+        omega = np.dot(t, s) / np.dot(t, t)
+        x = x + alpha * p + omega * s
+        if recalc and (it + 1) % recalc == 0:
+            r = b - A @ x
+        else:
+            r = s - omega * t
+        omega_prev, rho_prev = omega, rho
+        it += 1
     return x, it, hist

@@ -61,6 +61,16 @@ struct cpu_vector_operations {
     {
         for (size_t i = 0; i < sz_; ++i) y[i] = a * x[i] + b * y[i];
     }
+    // z = a x + b y
+    void assign_mul(T a, const vector_type& x, T b, const vector_type& y, vector_type& z) const
+    {
+        for (size_t i = 0; i < sz_; ++i) z[i] = a * x[i] + b * y[i];
+    }
+    // z = a x + b y + c z
+    void add_mul(T a, const vector_type& x, T b, const vector_type& y, T c, vector_type& z) const
+    {
+        for (size_t i = 0; i < sz_; ++i) z[i] = a * x[i] + b * y[i] + c * z[i];
+    }
 };
 
 // same operations plus the fused Gram-Schmidt pair of the concept extension

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "cpu_vector_operations.h"
+#include "lin_solvers/bicgstab.h"
 #include "lin_solvers/bicgstabl.h"
 #include "lin_solvers/default_monitor.h"
 #include "lin_solvers/gmres.h"
@@ -126,6 +127,20 @@ static void run_all(const char* tag, bool high)
         std::snprintf(name, sizeof name, "%s_bicgstabl%d", tag, L);
         report(name, vo, A, ok, s.monitor().iters_performed(), x, b, xd);
     }
+    for (int pc = 0; pc < 2; ++pc)
+        for (int pr = 0; pr < 2; ++pr)
+            for (int rf : {0, 5}) {
+                bicgstab<op_t, prec_t, VO, mon_t, hpb::log_std> s(&vo);
+                prec_t P;
+                if (pc) s.set_preconditioner(&P);
+                s.set_use_precond_resid(pr != 0);
+                s.set_resid_recalc_freq(rf);
+                s.monitor().init(1e-10, 0.0, 5000);
+                vo.assign_scalar(0.0, x);
+                bool ok = s.solve(A, b, x);
+                std::snprintf(name, sizeof name, "%s_bicgstab_prec%d_pr%d_rf%d", tag, pc, pr, rf);
+                report(name, vo, A, ok && !s.breakdown(), s.monitor().iters_performed(), x, b, xd);
+            }
     // failure handling: iteration cap, NaN in the right-hand side
     {
         gmres<op_t, prec_t, VO, mon_t, hpb::log_std, 100> s(&vo);

@@ -77,8 +77,26 @@ def test_bicgstabl_small(handle, L, dev):
     A.free()
 
 
+@pytest.mark.parametrize("recalc", [0, 1])
+@pytest.mark.parametrize("high", [0, 1])
+def test_bicgstab_small(handle, high, recalc):
+    Mh, A, b = _problem(handle, 16, 14, 12, S.CONVDIFF7)
+    say("bicgstab", high, recalc)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    p = S.params(solver=S.BICGSTAB, rel_tol=1e-12, max_iters=5000, high_precision=high, resid_recalc_freq=recalc)
+    res = S.solve(handle, A, db, dx, p)
+    assert res["converged"] == 1 and res["breakdown"] == 0
+    x = dx.numpy()
+    assert _host_rel(Mh, b, x) <= (1.5e-12 if recalc else 1e-11)
+    assert np.max(np.abs(x - spla.spsolve(Mh.tocsc(), b))) < 1e-9
+    xo, ito, hist = K.bicgstab(Mh, b, np.zeros_like(b), 1e-12, 5000, recalc=recalc)
+    assert abs(res["iters"] - ito) <= max(4, ito // 10), (res["iters"], ito)
+    A.free()
+
+
 @pytest.mark.parametrize("dev", [0, 1])
-@pytest.mark.parametrize("inner_solver,arg", [(S.GMRES, 30), (S.BICGSTABL, 2)])
+@pytest.mark.parametrize("inner_solver,arg", [(S.GMRES, 30), (S.BICGSTABL, 2), (S.BICGSTAB, 0)])
 def test_mixed_precision_refinement(handle, inner_solver, arg, dev):
     Mh, A, b = _problem(handle, 24, 24, 24, S.CONVDIFF7)
     say("ir", inner_solver, arg, dev)

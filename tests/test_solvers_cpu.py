@@ -1,8 +1,11 @@
 """The solver templates (cpp/lin_solvers) compiled with g++ against a host test
 double of the VectorOperations concept and run on the reference's 1-D upwind
 convection-diffusion problem (source/linear_operator.hpp:28-90): every variant
-converges to the tolerance, agrees with a direct tridiagonal solve, and the
-failure paths (iteration cap, NaN input) stop with the documented flags."""
+(GMRES MGS/CGS2, BiCGStab(L), plain BiCGStab with and without the
+preconditioner, preconditioned or plain residual, with and without residual
+recomputation) converges to the tolerance, agrees with a direct tridiagonal
+solve, and the failure paths (iteration cap, NaN input) stop with the documented
+flags."""
 import os
 import subprocess
 
@@ -28,8 +31,8 @@ def results(tmp_path_factory):
 
 
 def test_all_variants_converge(results):
-    solves = {k: v for k, v in results.items() if "gmres" in k or "bicgstabl" in k}
-    assert len(solves) == 21
+    solves = {k: v for k, v in results.items() if "gmres" in k or "bicgstab" in k}
+    assert len(solves) == 21 + 24
     for name, (conv, iters, rel, err) in solves.items():
         assert conv == "1", name
         assert float(rel) <= 1.5e-10, (name, rel)
