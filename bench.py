@@ -70,6 +70,30 @@ def gendot(H, h, n, cond, seed, x, y):
     return exact_value(limbs), dd
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_reduce_dot2_n2p27_traffic.md")
+
+
+def profiled_traffic(n, cfg, nontemporal):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
+    copy kernel of known size) for this exact launch, from the committed profile; None
+    when the profile does not cover this n and launch shape."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    fields = {}
+    for line in text.splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            fields["traffic"] = int(line.split(":")[1])
+        elif line.startswith("launch:"):
+            fields.update(kv.split("=") for kv in line.split(":", 1)[1].split())
+    want = {"n": n, "block": cfg[0], "unroll": cfg[1], "grid": cfg[2], "vector_width": cfg[3],
+            "nontemporal": int(bool(nontemporal))}
+    if "traffic" not in fields or any(int(fields.get(k, -1)) != v for k, v in want.items()):
+        return None
+    return fields["traffic"]
+
+
 def timed(h, fn, args, steps, warmup):
     for _ in range(warmup):
         rc = fn(*args)
@@ -499,6 +523,7 @@ def main():
     nbytes = 16.0 * n
     gbs = nbytes / (ms * 1e-3) / 1e9
     cfg = h.reduce_config("float64", n, True)
+    traffic = profiled_traffic(n, cfg, h.get_tuning(0, np.float64)[3] & 1)
     out = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -517,7 +542,8 @@ def main():
                    "launch": {"block": cfg[0], "unroll": cfg[1], "grid": cfg[2], "vector_width": cfg[3]}},
         "hbm_peak_frac": round(gbs / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                      "kernel": "hpb::k_reduce<double, DOT2> (hpb_dot2_f64)",
                      "algorithmic_bytes_per_launch": int(nbytes)},
         "result": {"dot2": float(res[2]), "exact": float(exact), "rel_err": rel_err(res[2], exact),

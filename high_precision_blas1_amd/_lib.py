@@ -24,6 +24,7 @@ _SCALARS = {
     "int64_t": ctypes.c_int64,
     "hpb_handle_t": ctypes.c_void_p,
     "hpb_batch_t": ctypes.c_void_p,
+    "hpb_trsv_t": ctypes.c_void_p,
 }
 
 _PROTO = re.compile(r"^(int|const char\*|void)\s+(hpb_\w+)\s*\(([^;]*?)\)\s*;", re.M | re.S)

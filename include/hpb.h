@@ -290,6 +290,34 @@ int hpb_lognormal_field_f64(hpb_handle_t h, int64_t ncell, double sigma, uint64_
  * diagonal = sum of the four faces ((s + w) + e) + n. */
 int hpb_csr_diffusion2d_f64(hpb_handle_t h, int m, const double* kappa, int* row_ptr, int* col_ind, double* val);
 
+/* ---- sparse triangular solves and ILU(0) ---------------------------------- */
+/* csr::gpu_matrix::triangular_solve_lower / _upper (common/csr/gpu_matrix.h:142-320,
+ * cuSPARSE csrsv2 with an analysis and a cudaMalloc per call).  The analysis is done
+ * once per pattern (levels of the dependency graph; the caller's row_ptr / col_ind
+ * must stay valid and unchanged while the analysis is used; columns sorted and
+ * unique in each row):
+ *   uplo 0: L x = alpha b with the strictly lower triangle and a unit diagonal
+ *           (the stored diagonal is ignored, CUSPARSE_DIAG_TYPE_UNIT);
+ *   uplo 1: U x = alpha b with the upper triangle and its stored diagonal (every
+ *           row must store one).
+ * One kernel per level; compensated = 1 evaluates each row's sum with Dot2. x may
+ * alias b. */
+typedef struct hpb_trsv_s* hpb_trsv_t;
+int hpb_csr_trsv_analysis(hpb_handle_t h, int uplo, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
+                          hpb_trsv_t* out);
+int hpb_csr_trsv_levels(hpb_trsv_t t, int* nlevels);
+int hpb_trsv_destroy(hpb_trsv_t t);
+int hpb_csr_trsv_f64(hpb_handle_t h, hpb_trsv_t t, const double* val, double alpha, const double* b, double* x,
+                     int compensated);
+int hpb_csr_trsv_f32(hpb_handle_t h, hpb_trsv_t t, const float* val, float alpha, const float* b, float* x,
+                     int compensated);
+/* ILU(0) in place on the pattern of A, using the uplo 0 analysis of that pattern:
+ * afterwards the strictly lower part holds L (unit diagonal implied) and the rest U,
+ * with (L U)_ij = a_ij on the pattern.  The reference's gpu_ilu0_preconditioner
+ * (common/csr/gpu_ilu0_preconditioner.h:62-72) does not factor; this one does.
+ * Every row must store its diagonal entry. */
+int hpb_csr_ilu0_f64(hpb_handle_t h, hpb_trsv_t lower, double* val);
+
 /* ---- Krylov solvers on CSR matrices ---------------------------------------- */
 /* The solver templates (gmres.h:65-495, bicgstabl.h:68-275 of the reference's
  * numerical_algos/lin_solvers, restated in high_precision_blas1_amd/cpp) over
