@@ -9,6 +9,10 @@
 // products (use_compensated_rows), and a fused residual r := b - A x is offered.
 // Transposed products are not supported (no solver in the path uses them) and
 // throw std::logic_error.
+// triangular_solve_lower / _upper (gpu_matrix.h:142-320) solve with the lower
+// triangle and a unit diagonal / the upper triangle and the stored diagonal; the
+// level analysis of each triangle is made on first use and kept until the
+// pattern changes (the reference redoes it on every call).
 #pragma once
 #include <cstdint>
 #include <stdexcept>
@@ -31,7 +35,11 @@ public:
     explicit hip_matrix(hpb_handle_t h) : h_(h) {}
     hip_matrix(const hip_matrix&) = delete;
     hip_matrix& operator=(const hip_matrix&) = delete;
-    ~hip_matrix() { release_(); }
+    ~hip_matrix()
+    {
+        drop_analyses_();
+        release_();
+    }
 
     void set_dim(int matrix_columns, int matrix_rows)
     {
@@ -56,6 +64,7 @@ public:
     void set(const T* data, const int* col_ind, const int* row_ptr)
     {
         if (!initialized_) throw std::runtime_error("csr::matrix: trying to set an uninitialized matrix.");
+        drop_analyses_();
         hpb::check(hpb_memcpy_h2d(h_, data_, data, sizeof(T) * (size_t)nnz_), "csr::hip_matrix::set");
         hpb::check(hpb_memcpy_h2d(h_, col_ind_, col_ind, sizeof(int) * (size_t)size_col_), "csr::hip_matrix::set");
         hpb::check(hpb_memcpy_h2d(h_, row_ptr_, row_ptr, sizeof(int) * (size_t)size_row_), "csr::hip_matrix::set");
@@ -72,6 +81,7 @@ public:
     void adopt_device(int rows, int cols, int64_t nnz, int* row_ptr, int* col_ind, T* data, bool own = true)
     {
         release_();
+        drop_analyses_();
         set_dim(cols, rows);
         nnz_ = nnz;
         size_col_ = nnz;
@@ -95,6 +105,24 @@ public:
                    "csr::hip_matrix::axpy");
     }
 
+    // L x = alpha b: strictly lower triangle, unit diagonal
+    void triangular_solve_lower(const vector_type& b, vector_type& x, const T& alpha = T(1.0),
+                                bool transpose = false) const
+    {
+        trsv_(0, b, x, alpha, transpose);
+    }
+    // U x = alpha b: upper triangle with the stored diagonal
+    void triangular_solve_upper(const vector_type& b, vector_type& x, const T& alpha = T(1.0),
+                                bool transpose = false) const
+    {
+        trsv_(1, b, x, alpha, transpose);
+    }
+    // in-place ILU(0) of the values (pattern kept): L strictly below, U on and above
+    void ilu0_in_place()
+    {
+        hpb::check(hpb::abi<T>::csr_ilu0(h_, analysis_(0), data_), "csr::hip_matrix::ilu0");
+    }
+
     int rows() const { return dim_.rows; }
     int64_t nnz() const { return nnz_; }
     const int* row_ptr() const { return row_ptr_; }
@@ -103,6 +131,26 @@ public:
     hpb_handle_t handle() const { return h_; }
 
 private:
+    hpb_trsv_t analysis_(int uplo) const
+    {
+        hpb_trsv_t& a = uplo == 0 ? lower_ : upper_;
+        if (!a)
+            hpb::check(hpb_csr_trsv_analysis(h_, uplo, dim_.rows, nnz_, row_ptr_, col_ind_, &a),
+                       "csr::hip_matrix: triangular analysis");
+        return a;
+    }
+    void trsv_(int uplo, const vector_type& b, vector_type& x, const T& alpha, bool transpose) const
+    {
+        if (transpose) throw std::logic_error("csr::hip_matrix: transposed triangular solve not supported");
+        hpb::check(hpb::abi<T>::csr_trsv(h_, analysis_(uplo), data_, alpha, b, x, comp_ ? 1 : 0),
+                   "csr::hip_matrix: triangular solve");
+    }
+    void drop_analyses_()
+    {
+        hpb_trsv_destroy(lower_);
+        hpb_trsv_destroy(upper_);
+        lower_ = upper_ = nullptr;
+    }
     void release_()
     {
         if (own_) {
@@ -125,6 +173,7 @@ private:
     bool initialized_ = false;
     bool comp_ = false;
     bool own_ = true;
+    mutable hpb_trsv_t lower_ = nullptr, upper_ = nullptr;
 };
 
 }  // namespace csr

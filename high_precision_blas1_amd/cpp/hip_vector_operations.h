@@ -44,7 +44,7 @@ template <> struct abi<double> {
     HPB_F(axpbypcwpdz) HPB_F(assign_mul4) HPB_F(mul_pointwise) HPB_F(mul_pointwise_inplace) HPB_F(div_pointwise)
     HPB_F(div_pointwise_inplace) HPB_F(axpbyc) HPB_F(valid) HPB_F(set_value) HPB_F(get_value)
     HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc) HPB_F(multidot) HPB_F(multiaxpy) HPB_FA(multidot)
-    HPB_F(multiaxpy_dev)
+    HPB_F(multiaxpy_dev) HPB_F(csr_trsv) HPB_F(csr_ilu0)
 #undef HPB_F
 #undef HPB_FA
 };
@@ -57,7 +57,7 @@ template <> struct abi<float> {
     HPB_F(axpbypcwpdz) HPB_F(assign_mul4) HPB_F(mul_pointwise) HPB_F(mul_pointwise_inplace) HPB_F(div_pointwise)
     HPB_F(div_pointwise_inplace) HPB_F(axpbyc) HPB_F(valid) HPB_F(set_value) HPB_F(get_value)
     HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc) HPB_F(multidot) HPB_F(multiaxpy) HPB_FA(multidot)
-    HPB_F(multiaxpy_dev)
+    HPB_F(multiaxpy_dev) HPB_F(csr_trsv) HPB_F(csr_ilu0)
 #undef HPB_F
 #undef HPB_FA
 };

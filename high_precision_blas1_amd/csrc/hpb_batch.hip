@@ -71,6 +71,13 @@ struct batch_t {
     double* part = nullptr;         // ntiles * BQ * 2
     double* dots = nullptr;         // nsys * BQ
     int* hcount = nullptr;          // pinned
+    int* tile_list = nullptr;       // ntiles: indices of the tiles of active systems
+    int* tile_count = nullptr;      // 1: entries of tile_list in use
+    double* dinv = nullptr;         // nrows: 1 / a_ii (Jacobi right preconditioning)
+    int jacobi = 0;
+    double graph_tol2 = -1.0;       // settings the captured graph was recorded with
+    int graph_jacobi = -1;
+    int grid = 1;                   // blocks of the tile kernels (grid-stride over tile_list)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int verbose = 0;                // print the active count at every check
@@ -143,29 +150,38 @@ __global__ __launch_bounds__(256) void k_batch_build(int nsys, int64_t nrows, co
 
 // ---------------------------------------------------------------- kernels over tiles
 
-// y = A x (MODE 0) or y = b - A x (MODE 1, compensated rows) for active systems
+// y = A D x (MODE 0) or y = b - A D x (MODE 1, compensated rows) for active
+// systems; D = diag(dinv) (Jacobi right preconditioning) or I when dinv is null.
+// The grid strides over the list of tiles of active systems.
 template <int MODE>
-__global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ tiles, const int* __restrict__ active,
+__global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ tiles, const int* __restrict__ list,
+                                                      const int* __restrict__ count, const int* __restrict__ active,
                                                       const int* __restrict__ rp, const int* __restrict__ col,
-                                                      const double* __restrict__ val, const double* __restrict__ x,
-                                                      const double* __restrict__ b, double* __restrict__ y)
+                                                      const double* __restrict__ val, const double* __restrict__ dinv,
+                                                      const double* __restrict__ x, const double* __restrict__ b,
+                                                      double* __restrict__ y)
 {
-    const btile t = tiles[blockIdx.x];
-    if (!active[t.sys]) return;
-    constexpr int LPR = 4;
-    const int lane = threadIdx.x % LPR;
-    for (int64_t r = t.r0 + threadIdx.x / LPR; r < t.r1; r += BT_THREADS / LPR) {
-        double s = (MODE == 1 && lane == 0) ? b[r] : 0.0, c = 0.0;
-        for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
-            const double a = MODE == 1 ? -val[k] : val[k];
-            dev::dot2_step<double>(s, c, a, x[col[k]]);
-        }
+    const int nt = *count;
+    for (int q = blockIdx.x; q < nt; q += gridDim.x) {
+        const btile t = tiles[list[q]];
+        if (!active[t.sys]) continue;
+        constexpr int LPR = 4;
+        const int lane = threadIdx.x % LPR;
+        for (int64_t r = t.r0 + threadIdx.x / LPR; r < t.r1; r += BT_THREADS / LPR) {
+            double s = (MODE == 1 && lane == 0) ? b[r] : 0.0, c = 0.0;
+            for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
+                const double a = MODE == 1 ? -val[k] : val[k];
+                const int j = col[k];
+                const double xv = dinv ? x[j] * dinv[j] : x[j];
+                dev::dot2_step<double>(s, c, a, xv);
+            }
 #pragma unroll
-        for (int m = LPR / 2; m >= 1; m >>= 1) {
-            double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
-            dev::merge<double>(s, c, s2, c2);
+            for (int m = LPR / 2; m >= 1; m >>= 1) {
+                double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
+                dev::merge<double>(s, c, s2, c2);
+            }
+            if (lane == 0) y[r] = s + c;
         }
-        if (lane == 0) y[r] = s + c;
     }
 }
 
@@ -174,26 +190,31 @@ struct dot_pairs {
     const double* b[BQ];
 };
 
-// per-tile compensated partials of NQ dot products
+// per-tile compensated partials of NQ dot products (tiles of active systems)
 template <int NQ>
-__global__ __launch_bounds__(BT_THREADS) void k_bdots(const btile* __restrict__ tiles, const int* __restrict__ active,
+__global__ __launch_bounds__(BT_THREADS) void k_bdots(const btile* __restrict__ tiles, const int* __restrict__ list,
+                                                      const int* __restrict__ count, const int* __restrict__ active,
                                                       dot_pairs P, double* __restrict__ part)
 {
-    const btile t = tiles[blockIdx.x];
-    if (!active[t.sys]) return;
-    double s[NQ], c[NQ];
+    const int nt = *count;
+    for (int q = blockIdx.x; q < nt; q += gridDim.x) {
+        const int ti = list[q];
+        const btile t = tiles[ti];
+        if (!active[t.sys]) continue;
+        double s[NQ], c[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) s[q] = c[q] = 0.0;
-    for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += BT_THREADS) {
+        for (int k = 0; k < NQ; ++k) s[k] = c[k] = 0.0;
+        for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += BT_THREADS) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) dev::dot2_step<double>(s[q], c[q], P.a[q][r], P.b[q][r]);
-    }
+            for (int k = 0; k < NQ; ++k) dev::dot2_step<double>(s[k], c[k], P.a[k][r], P.b[k][r]);
+        }
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        dev::block_reduce<double, true, BT_THREADS>(s[q], c[q]);
-        if (threadIdx.x == 0) {
-            part[((size_t)blockIdx.x * BQ + q) * 2] = s[q];
-            part[((size_t)blockIdx.x * BQ + q) * 2 + 1] = c[q];
+        for (int k = 0; k < NQ; ++k) {
+            dev::block_reduce<double, true, BT_THREADS>(s[k], c[k]);
+            if (threadIdx.x == 0) {
+                part[((size_t)ti * BQ + k) * 2] = s[k];
+                part[((size_t)ti * BQ + k) * 2 + 1] = c[k];
+            }
         }
     }
 }
@@ -274,14 +295,17 @@ __global__ __launch_bounds__(256) void k_bsys(int nsys, const int* __restrict__ 
 
 // updates; every tile reads its system's scalars
 template <int KIND>
-__global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ tiles, const int* __restrict__ active,
+__global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ tiles, const int* __restrict__ list,
+                                                     const int* __restrict__ count, const int* __restrict__ active,
                                                      const double* __restrict__ st, double* __restrict__ x,
                                                      double* __restrict__ r0, double* __restrict__ r1,
                                                      double* __restrict__ r2, double* __restrict__ u0,
                                                      double* __restrict__ u1, double* __restrict__ u2)
 {
-    const btile t = tiles[blockIdx.x];
-    if (!active[t.sys]) return;
+  const int nt = *count;
+  for (int q = blockIdx.x; q < nt; q += gridDim.x) {
+    const btile t = tiles[list[q]];
+    if (!active[t.sys]) continue;
     const double* S = st + (size_t)t.sys * S_NSTATE;
     const double beta = S[S_BETA], alpha = S[S_ALPHA];
     for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += BT_THREADS) {
@@ -306,6 +330,56 @@ __global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ t
             u0[r] = dev::fma_rn<double>(-g1, u1[r], dev::fma_rn<double>(-g2, u2[r], u0[r]));
         }
     }
+  }
+}
+
+// list of the tiles of active systems (one block; systems in order, so the list
+// is ordered and the per-system merges see their tiles in tile order)
+__global__ __launch_bounds__(256) void k_bcompact(int nsys, const int* __restrict__ tile_first,
+                                                  const int* __restrict__ active, int* __restrict__ list,
+                                                  int* __restrict__ count)
+{
+    __shared__ int base;
+    __shared__ int scan[256];
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (int s0 = 0; s0 < nsys; s0 += 256) {
+        const int s = s0 + threadIdx.x;
+        const int nt = (s < nsys && active[s]) ? tile_first[s + 1] - tile_first[s] : 0;
+        scan[threadIdx.x] = nt;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {   // inclusive Hillis-Steele scan
+            const int v = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0;
+            __syncthreads();
+            scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const int start = base + scan[threadIdx.x] - nt;
+        for (int k = 0; k < nt; ++k) list[start + k] = tile_first[s] + k;
+        __syncthreads();
+        if (threadIdx.x == 255) base += scan[255];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = base;
+}
+
+// dinv_i = 1 / a_ii
+__global__ __launch_bounds__(256) void k_bdiag(int64_t nrows, const int* __restrict__ rp, const int* __restrict__ col,
+                                               const double* __restrict__ val, double* __restrict__ dinv)
+{
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * 256) {
+        double d = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k)
+            if (col[k] == (int)r) d = val[k];
+        dinv[r] = 1.0 / d;
+    }
+}
+
+// x := dinv * x (the solution of the preconditioned system back to x)
+__global__ __launch_bounds__(256) void k_bscale(int64_t nrows, const double* __restrict__ dinv, double* __restrict__ x)
+{
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * 256)
+        x[r] = x[r] * dinv[r];
 }
 
 __global__ void k_zero_int(int* p) { *p = 0; }
@@ -318,7 +392,7 @@ static int free_batch(batch_t* B)
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
     void* ptrs[] = {B->d_tiles, B->d_tile_first, B->d_row_off, B->rp, B->col, B->val, B->st, B->active,
-                    B->iters, B->nactive, B->part, B->dots};
+                    B->iters, B->nactive, B->part, B->dots, B->tile_list, B->tile_count, B->dinv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (double* p : B->v)
@@ -334,21 +408,24 @@ static unsigned sys_blocks(int nsys) { return (unsigned)((nsys * 64 + 255) / 256
 static void record_cycle(batch_t* B, double tol2)
 {
     hipStream_t st = B->h->stream;
-    const unsigned T = (unsigned)B->ntiles, SB = sys_blocks(B->nsys);
+    const unsigned G = (unsigned)B->grid, SB = sys_blocks(B->nsys);
     double** v = B->v;
+    const double* dinv = B->jacobi ? B->dinv : nullptr;
     auto dots1 = [&](const double* a, const double* b) {
         dot_pairs P{};
         P.a[0] = a;
         P.b[0] = b;
-        hipLaunchKernelGGL((k_bdots<1>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+        hipLaunchKernelGGL((k_bdots<1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                           B->active, P, B->part);
     };
     auto spmv = [&](const double* x, double* y) {
-        hipLaunchKernelGGL((k_bspmv<0>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, B->rp, B->col,
-                           B->val, x, (const double*)nullptr, y);
+        hipLaunchKernelGGL((k_bspmv<0>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                           B->active, B->rp, B->col, B->val, dinv, x, (const double*)nullptr, y);
     };
     auto upd = [&](auto kind) {
-        hipLaunchKernelGGL((k_bupd<decltype(kind)::value>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active,
-                           B->st, v[VX], v[VR0], v[VR1], v[VR2], v[VU0], v[VU1], v[VU2]);
+        hipLaunchKernelGGL((k_bupd<decltype(kind)::value>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles,
+                           B->tile_list, B->tile_count, B->active, B->st, v[VX], v[VR0], v[VR1], v[VR2], v[VU0],
+                           v[VU1], v[VU2]);
     };
     auto sys = [&](auto nq, auto phase, int first) {
         hipLaunchKernelGGL((k_bsys<decltype(nq)::value, decltype(phase)::value>), dim3(SB), dim3(256), 0, st, B->nsys,
@@ -365,20 +442,20 @@ static void record_cycle(batch_t* B, double tol2)
     dots1(v[VR0], v[VRT]);
     sys(I1{}, I0{}, 1);
     upd(I0{});                       // u0 = r0 - beta u0
-    spmv(v[VU0], v[VU1]);            // u1 = A u0
+    spmv(v[VU0], v[VU1]);            // u1 = A D u0
     dots1(v[VU1], v[VRT]);
     sys(I1{}, I1{}, 0);              // alpha
     upd(I1{});                       // r0 -= alpha u1, x += alpha u0
-    spmv(v[VR0], v[VR1]);            // r1 = A r0
+    spmv(v[VR0], v[VR1]);            // r1 = A D r0
     // j = 1
     dots1(v[VR1], v[VRT]);
     sys(I1{}, I0{}, 0);
     upd(I2{});                       // u0, u1
-    spmv(v[VU1], v[VU2]);            // u2 = A u1
+    spmv(v[VU1], v[VU2]);            // u2 = A D u1
     dots1(v[VU2], v[VRT]);
     sys(I1{}, I1{}, 0);
     upd(I3{});                       // r0, r1, x
-    spmv(v[VR1], v[VR2]);            // r2 = A r1
+    spmv(v[VR1], v[VR2]);            // r2 = A D r1
     // MR part
     {
         dot_pairs P{};
@@ -387,15 +464,19 @@ static void record_cycle(batch_t* B, double tol2)
         P.a[2] = v[VR2]; P.b[2] = v[VR2];
         P.a[3] = v[VR0]; P.b[3] = v[VR1];
         P.a[4] = v[VR0]; P.b[4] = v[VR2];
-        hipLaunchKernelGGL((k_bdots<5>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+        hipLaunchKernelGGL((k_bdots<5>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                           B->active, P, B->part);
     }
     sys(I5{}, I2{}, 0);
     upd(I4{});
-    // true residual r0 = b - A x and the convergence check
-    hipLaunchKernelGGL((k_bspmv<1>), dim3(T), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, B->rp, B->col, B->val,
-                       v[VX], v[VB], v[VR0]);
+    // true residual r0 = b - A D x and the convergence check
+    hipLaunchKernelGGL((k_bspmv<1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                       B->active, B->rp, B->col, B->val, dinv, v[VX], v[VB], v[VR0]);
     dots1(v[VR0], v[VR0]);
     sys(I1{}, I3{}, 0);
+    // next cycle only visits the tiles of systems still active
+    hipLaunchKernelGGL(k_bcompact, dim3(1), dim3(256), 0, st, B->nsys, B->d_tile_first, B->active, B->tile_list,
+                       B->tile_count);
 }
 
 }  // namespace hpb
@@ -454,6 +535,9 @@ int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, dou
     M((void**)&B->nactive, sizeof(int));
     M((void**)&B->part, sizeof(double) * 2 * BQ * B->ntiles);
     M((void**)&B->dots, sizeof(double) * BQ * nsys);
+    M((void**)&B->tile_list, sizeof(int) * B->ntiles);
+    M((void**)&B->tile_count, sizeof(int));
+    M((void**)&B->dinv, sizeof(double) * B->nrows);
     if (e == hipSuccess) e = hipHostMalloc((void**)&B->hcount, sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_batch(B);
@@ -490,6 +574,16 @@ int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, dou
         free_batch(B);
         return rc;
     }
+    {
+        int64_t g2 = (B->nrows + 255) / 256;
+        if (g2 > cap) g2 = cap;
+        hipLaunchKernelGGL(k_bdiag, dim3((unsigned)g2), dim3(256), 0, st, B->nrows, B->rp, B->col, B->val, B->dinv);
+        HPB_LAUNCH_CHECK();
+    }
+    {
+        const int cu = h->num_cu > 0 ? h->num_cu : 256;
+        B->grid = std::min(B->ntiles, 8 * cu);
+    }
     HPB_HIP(hipStreamSynchronize(st));
     *out = (hpb_batch_t)B;
     return HPB_OK;
@@ -518,7 +612,18 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     const double tol2 = rel_tol * rel_tol;
     double** v = B->v;
     const size_t nb = sizeof(double) * B->nrows;
-    // x = 0, r0 = b, rt = r0 (unnormalised: scaling r~ rescales rho and (u, r~) alike), u0 = 0
+    const unsigned G = (unsigned)B->grid;
+    auto all_active = [&]() -> int {
+        std::vector<int> ones(B->nsys, 1);
+        HPB_HIP(hipMemcpyAsync(B->active, ones.data(), sizeof(int) * B->nsys, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_bcompact, dim3(1), dim3(256), 0, st, B->nsys, B->d_tile_first, B->active, B->tile_list,
+                           B->tile_count);
+        HPB_LAUNCH_CHECK();
+        HPB_HIP(hipStreamSynchronize(st));
+        return HPB_OK;
+    };
+    // x = 0 (x^ = D^-1 x with Jacobi), r0 = b, rt = r0 (unnormalised: scaling r~
+    // rescales rho and (u, r~) alike), u0 = 0
     HPB_HIP(hipMemsetAsync(v[VX], 0, nb, st));
     HPB_HIP(hipMemsetAsync(v[VU0], 0, nb, st));
     HPB_HIP(hipMemsetAsync(v[VU1], 0, nb, st));
@@ -526,21 +631,27 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     HPB_HIP(hipMemcpyAsync(v[VR0], v[VB], nb, hipMemcpyDeviceToDevice, st));
     HPB_HIP(hipMemcpyAsync(v[VRT], v[VB], nb, hipMemcpyDeviceToDevice, st));
     HPB_HIP(hipMemsetAsync(B->iters, 0, sizeof(int) * B->nsys, st));
-    {
-        std::vector<int> ones(B->nsys, 1);
-        HPB_HIP(hipMemcpyAsync(B->active, ones.data(), sizeof(int) * B->nsys, hipMemcpyHostToDevice, st));
-        HPB_HIP(hipStreamSynchronize(st));
-    }
+    if (int rc = all_active()) return rc;
     {
         dot_pairs P{};
         P.a[0] = v[VB];
         P.b[0] = v[VB];
         P.a[1] = v[VR0];
         P.b[1] = v[VR0];
-        hipLaunchKernelGGL((k_bdots<2>), dim3(B->ntiles), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+        hipLaunchKernelGGL((k_bdots<2>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                           B->active, P, B->part);
         hipLaunchKernelGGL((k_bsys<2, 4>), dim3(sys_blocks(B->nsys)), dim3(256), 0, st, B->nsys, B->d_tile_first,
                            B->part, B->st, B->active, B->iters, B->nactive, tol2, 0);
+        hipLaunchKernelGGL(k_bcompact, dim3(1), dim3(256), 0, st, B->nsys, B->d_tile_first, B->active, B->tile_list,
+                           B->tile_count);
         HPB_LAUNCH_CHECK();
+    }
+    // the captured cycle bakes in the tolerance and the preconditioner choice
+    if (B->exec && (B->graph_tol2 != tol2 || B->graph_jacobi != B->jacobi)) {
+        (void)hipGraphExecDestroy(B->exec);
+        (void)hipGraphDestroy(B->graph);
+        B->exec = nullptr;
+        B->graph = nullptr;
     }
     if (use_graph && !B->exec) {
         HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -548,6 +659,8 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
         hipError_t ce = hipStreamEndCapture(st, &B->graph);
         if (ce != hipSuccess) return hpb::hip_status(ce, __FILE__, __LINE__, "hipStreamEndCapture");
         HPB_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        B->graph_tol2 = tol2;
+        B->graph_jacobi = B->jacobi;
     }
     HPB_HIP(hipStreamSynchronize(st));
     auto t0 = std::chrono::steady_clock::now();
@@ -568,19 +681,28 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
                          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         if (*B->hcount == 0) break;
     }
+    // back from the preconditioned unknowns: x = D^-1 x^
+    if (B->jacobi) {
+        const int cu = h->num_cu > 0 ? h->num_cu : 256;
+        int64_t g = (B->nrows + 255) / 256;
+        if (g > (int64_t)cu * 16) g = (int64_t)cu * 16;
+        hipLaunchKernelGGL(k_bscale, dim3((unsigned)g), dim3(256), 0, st, B->nrows, B->dinv, v[VX]);
+        HPB_LAUNCH_CHECK();
+    }
+    HPB_HIP(hipStreamSynchronize(st));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    // per-system true relative residuals, compensated
-    std::vector<int> ones(B->nsys, 1);
-    HPB_HIP(hipMemcpyAsync(B->active, ones.data(), sizeof(int) * B->nsys, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL((k_bspmv<1>), dim3(B->ntiles), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, B->rp, B->col,
-                       B->val, v[VX], v[VB], v[VR1]);
+    // per-system true relative residuals ||b - A x|| / ||b|| of the original systems, compensated
+    if (int rc = all_active()) return rc;
+    hipLaunchKernelGGL((k_bspmv<1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                       B->active, B->rp, B->col, B->val, (const double*)nullptr, v[VX], v[VB], v[VR1]);
     {
         dot_pairs P{};
         P.a[0] = v[VR1];
         P.b[0] = v[VR1];
         P.a[1] = v[VB];
         P.b[1] = v[VB];
-        hipLaunchKernelGGL((k_bdots<2>), dim3(B->ntiles), dim3(BT_THREADS), 0, st, B->d_tiles, B->active, P, B->part);
+        hipLaunchKernelGGL((k_bdots<2>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                           B->active, P, B->part);
     }
     HPB_LAUNCH_CHECK();
     std::vector<double> part((size_t)B->ntiles * BQ * 2);
@@ -601,6 +723,13 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     }
     if (out_seconds) *out_seconds = secs;
     if (out_cycles) *out_cycles = cycles;
+    return HPB_OK;
+}
+
+int hpb_batch_set_jacobi(hpb_batch_t b, int on)
+{
+    HPB_REQUIRE(b, HPB_ERR_INVALID, "hpb_batch_set_jacobi: null batch");
+    ((batch_t*)b)->jacobi = on ? 1 : 0;
     return HPB_OK;
 }
 

@@ -15,7 +15,7 @@
 //
 // Passes: k_head (parallel, per-block exact partials) -> k_head_reduce (one block)
 // -> k_tail_chunks (one thread per chunk, exact chunk partials) -> k_tail_fix
-// (one thread, sequential over chunks) -> results.
+// (one wave; the chain over chunks is sequential) -> results.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -207,41 +207,70 @@ __global__ __launch_bounds__(GD_BLOCK) void k_tail_chunks(uint64_t n_head, uint6
 
 // Sequential pass over the chunks: the last element of every chunk cancels the
 // exact running sum.  acc[0..7] holds the head sums on entry and the totals on exit.
-__global__ void k_tail_fix(uint64_t n_head, uint64_t n_tail, const uint64_t* __restrict__ cpart,
-                           const double* __restrict__ r_last, double* __restrict__ x, double* __restrict__ y,
-                           uint64_t* __restrict__ acc, double* __restrict__ out3)
+// The 64 lanes stage the inputs of FIX_BATCH chunks in LDS (independent loads in
+// parallel), lane 0 runs the dependent chain from LDS, then the lanes store the
+// new y values; the arithmetic is the sequential one.
+constexpr int FIX_BATCH = 64;
+
+__global__ __launch_bounds__(64) void k_tail_fix(uint64_t n_head, uint64_t n_tail, const uint64_t* __restrict__ cpart,
+                                                 const double* __restrict__ r_last, double* __restrict__ x,
+                                                 double* __restrict__ y, uint64_t* __restrict__ acc,
+                                                 double* __restrict__ out3)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ uint64_t sp[FIX_BATCH * 8];
+    __shared__ double sx[FIX_BATCH], sr[FIX_BATCH], sy[FIX_BATCH];
     acc256 S, A;
     for (int k = 0; k < 4; ++k) {
         S.w[k] = acc[k];
         A.w[k] = acc[4 + k];
     }
     const uint64_t nchunks = (n_tail + GD_CHUNK - 1) / GD_CHUNK;
-    for (uint64_t ch = 0; ch < nchunks; ++ch) {
-        acc256 c, a;
-        for (int k = 0; k < 4; ++k) {
-            c.w[k] = cpart[ch * 8 + k];
-            a.w[k] = cpart[ch * 8 + 4 + k];
+    const int lane = threadIdx.x;
+    for (uint64_t c0 = 0; c0 < nchunks; c0 += FIX_BATCH) {
+        const int nb = (int)((nchunks - c0) < (uint64_t)FIX_BATCH ? (nchunks - c0) : (uint64_t)FIX_BATCH);
+        for (int q = lane; q < nb * 8; q += 64) sp[q] = cpart[c0 * 8 + q];
+        if (lane < nb) {
+            const uint64_t ch = c0 + lane;
+            uint64_t j1 = (ch + 1) * GD_CHUNK;
+            if (j1 > n_tail) j1 = n_tail;
+            sx[lane] = x[n_head + j1 - 1];
+            sr[lane] = r_last[ch];
         }
-        acc_add(S, c);
-        acc_add(A, a);
-        uint64_t j1 = (ch + 1) * GD_CHUNK;
-        if (j1 > n_tail) j1 = n_tail;
-        const uint64_t i = n_head + j1 - 1;
-        const double xi = x[i];
-        const double yi = gd_grid53((r_last[ch] - acc_round(S)) / xi);
-        y[i] = yi;
-        gd_add_prod(S, A, xi, yi);
+        __syncthreads();
+        if (lane == 0) {
+            for (int b = 0; b < nb; ++b) {
+                acc256 c, a;
+                for (int k = 0; k < 4; ++k) {
+                    c.w[k] = sp[b * 8 + k];
+                    a.w[k] = sp[b * 8 + 4 + k];
+                }
+                acc_add(S, c);
+                acc_add(A, a);
+                const double xi = sx[b];
+                const double yi = gd_grid53((sr[b] - acc_round(S)) / xi);
+                sy[b] = yi;
+                gd_add_prod(S, A, xi, yi);
+            }
+        }
+        __syncthreads();
+        if (lane < nb) {
+            const uint64_t ch = c0 + lane;
+            uint64_t j1 = (ch + 1) * GD_CHUNK;
+            if (j1 > n_tail) j1 = n_tail;
+            y[n_head + j1 - 1] = sy[lane];
+        }
+        __syncthreads();
     }
-    for (int k = 0; k < 4; ++k) {
-        acc[k] = S.w[k];
-        acc[4 + k] = A.w[k];
+    if (lane == 0) {
+        for (int k = 0; k < 4; ++k) {
+            acc[k] = S.w[k];
+            acc[4 + k] = A.w[k];
+        }
+        const double d = acc_round(S), ab = acc_round(A);
+        out3[0] = d;
+        out3[1] = ab;
+        out3[2] = d != 0.0 ? ab / fabs(d) : INFINITY;
     }
-    const double d = acc_round(S), ab = acc_round(A);
-    out3[0] = d;
-    out3[1] = ab;
-    out3[2] = d != 0.0 ? ab / fabs(d) : INFINITY;
 }
 
 }  // namespace hpb

@@ -14,12 +14,15 @@
 
 #include <chrono>
 #include <cmath>
+#include <memory>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "csr/hip_matrix.h"
 #include "csr/linear_operator.h"
+#include "csr/preconditioner.h"
 #include "hip_vector_operations.h"
 #include "hpb_internal.hpp"
 #include "lin_solvers/bicgstab.h"
@@ -31,12 +34,13 @@ namespace {
 
 using namespace numerical_algos::lin_solvers;
 
-template <typename T>
+template <typename T, class Prec = void>
 struct solver_types {
     using vo_t = hpb::hip_vector_operations<T>;
     using mat_t = csr::hip_matrix<T>;
     using op_t = csr::linear_operator<vo_t, mat_t>;
-    using prec_t = identity_preconditioner<vo_t, op_t>;
+    using prec_t = std::conditional_t<std::is_void<Prec>::value, identity_preconditioner<vo_t, op_t>, Prec>;
+    using ilu_t = csr::ilu0_preconditioner<vo_t, op_t, mat_t>;
     using mon_t = default_monitor<vo_t, hpb::log_std>;
     using gmres_t = gmres<op_t, prec_t, vo_t, mon_t, hpb::log_std, 256>;
     using bicg_t = bicgstabl<op_t, prec_t, vo_t, mon_t, hpb::log_std>;
@@ -60,12 +64,16 @@ double true_rel_residual(hpb_handle_t h, const csr::hip_matrix<double>& A, const
     return ob[3] > 0 ? o[3] / ob[3] : o[3];
 }
 
-template <typename T>
+// prec: null for the unpreconditioned solvers, else the preconditioner object
+// (applied on the left, gmres.h:396-405; the monitor then sees P r when
+// use_precond_resid is set, as in the reference)
+template <typename T, class Prec = void>
 bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename solver_types<T>::vo_t& vo,
                 const typename solver_types<T>::op_t& op, const T* b, T* x, hpb_solver_result* res,
-                double* history, int history_cap, double rel_tol)
+                double* history, int history_cap, double rel_tol,
+                typename solver_types<T, Prec>::prec_t* prec = nullptr)
 {
-    using st = solver_types<T>;
+    using st = solver_types<T, Prec>;
     bool ok;
     auto fill = [&](const typename st::mon_t& mon) {
         res->iters = mon.iters_performed();
@@ -81,6 +89,8 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
     };
     if (p.solver == HPB_SOLVER_GMRES) {
         typename st::gmres_t s(&vo, log, 1);
+        if (prec) s.set_preconditioner(prec);
+        s.set_use_precond_resid(p.use_precond_resid != 0);
         s.set_restarts((unsigned)p.restart);
         s.set_gram_schmidt(p.gram_schmidt == 1 ? gram_schmidt::cgs2 : gram_schmidt::mgs);
         s.set_early_exit(p.early_exit != 0);
@@ -91,6 +101,8 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
         res->breakdown = 0;
     } else if (p.solver == HPB_SOLVER_BICGSTAB) {
         typename st::bicg1_t s(&vo, log, 1);
+        if (prec) s.set_preconditioner(prec);
+        s.set_use_precond_resid(p.use_precond_resid != 0);
         s.set_resid_recalc_freq(p.resid_recalc_freq);
         s.monitor().init((T)rel_tol, (T)p.abs_tol, p.max_iters, p.min_iters, false, p.save_history != 0, true);
         ok = s.solve(op, (T*)b, x);
@@ -98,6 +110,8 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
         res->breakdown = s.breakdown() ? 1 : 0;
     } else {
         typename st::bicg_t s(&vo, log, 1);
+        if (prec) s.set_preconditioner(prec);
+        s.set_use_precond_resid(p.use_precond_resid != 0);
         s.set_basis_size(p.basis);
         s.set_resid_recalc_freq(p.resid_recalc_freq);
         s.monitor().init((T)rel_tol, (T)p.abs_tol, p.max_iters, p.min_iters, false, p.save_history != 0, true);
@@ -111,7 +125,7 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
 // the device-resident path covers CGS2 GMRES and BiCGStab(L <= 8)
 bool use_device(const hpb_solver_params& p)
 {
-    if (!p.device_resident) return false;
+    if (!p.device_resident || p.preconditioner != HPB_PRECOND_NONE) return false;
     if (p.solver == HPB_SOLVER_GMRES) return p.gram_schmidt == 1 && !p.reorthogonalization;
     if (p.solver == HPB_SOLVER_BICGSTAB) return false;
     return p.basis <= 8;
@@ -125,6 +139,7 @@ int validate(const hpb_solver_params* p)
     if (p->solver == HPB_SOLVER_GMRES && (p->restart < 1 || p->restart >= 256)) return HPB_ERR_INVALID;
     if (p->solver == HPB_SOLVER_BICGSTABL && (p->basis < 1 || p->basis > 30)) return HPB_ERR_INVALID;
     if (p->max_iters < 1 || !(p->rel_tol >= 0)) return HPB_ERR_INVALID;
+    if (p->preconditioner != HPB_PRECOND_NONE && p->preconditioner != HPB_PRECOND_ILU0) return HPB_ERR_INVALID;
     return HPB_OK;
 }
 
@@ -151,6 +166,8 @@ void hpb_solver_params_default(hpb_solver_params* p)
     p->save_history = 0;
     p->verbose = 0;
     p->device_resident = 1;
+    p->preconditioner = HPB_PRECOND_NONE;
+    p->use_precond_resid = 1;
 }
 
 int hpb_csr_solve_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
@@ -177,6 +194,11 @@ int hpb_csr_solve_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr,
             const int rc = hpb::dsolve<double>(h, rows, nnz, row_ptr, col_ind, val, b, x, *p, res, history, history_cap);
             if (rc != HPB_OK) return rc;
             ok = res->converged != 0;
+        } else if (p->preconditioner == HPB_PRECOND_ILU0) {
+            typename st::ilu_t ilu(&vo);   // factored inside the timed region
+            ilu.set_matrix(A);
+            ok = run_solver<double, typename st::ilu_t>(*p, &log, vo, op, b, x, res, history, history_cap,
+                                                       p->rel_tol, &ilu);
         } else {
             ok = run_solver<double>(*p, &log, vo, op, b, x, res, history, history_cap, p->rel_tol);
         }
@@ -251,6 +273,7 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
             hpb::dsolver_base* c = nullptr;
             ~cache_guard() { hpb::dsolve_release(c); }
         } inner_cache;
+        std::unique_ptr<s32::ilu_t> ilu32;
         double ob[4];
         hpb::check(hpb_nrm2c_f64(h, n, b, ob), "ir: rhs norm");
         const double bnorm = ob[3];
@@ -277,6 +300,13 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
                 const int rc = hpb::dsolve<float>(h, rows, nnz, row_ptr, col_ind, val32, r32, d32, *inner, &ir,
                                                   nullptr, 0, &inner_cache.c);
                 if (rc != HPB_OK) return rc;
+            } else if (inner->preconditioner == HPB_PRECOND_ILU0) {
+                if (!ilu32) {
+                    ilu32.reset(new s32::ilu_t(&vo32));
+                    ilu32->set_matrix(A32);
+                }
+                run_solver<float, s32::ilu_t>(*inner, &log, vo32, op32, r32, d32, &ir, nullptr, 0, inner->rel_tol,
+                                              ilu32.get());
             } else {
                 run_solver<float>(*inner, &log, vo32, op32, r32, d32, &ir, nullptr, 0, inner->rel_tol);
             }

@@ -273,5 +273,9 @@ int hpb_csr_ilu0_f64(hpb_handle_t h, hpb_trsv_t lower, double* val)
 {
     return ilu0_entry<double>(h, lower, val);
 }
+int hpb_csr_ilu0_f32(hpb_handle_t h, hpb_trsv_t lower, float* val)
+{
+    return ilu0_entry<float>(h, lower, val);
+}
 
 }  // extern "C"

@@ -14,6 +14,8 @@ from . import DeviceArray, call
 GMRES = 0
 BICGSTABL = 1
 BICGSTAB = 2
+PRECOND_NONE = 0
+PRECOND_ILU0 = 1
 
 
 class SolverParams(ctypes.Structure):
@@ -22,7 +24,8 @@ class SolverParams(ctypes.Structure):
                 ("abs_tol", ctypes.c_double), ("high_precision", ctypes.c_int), ("compensated_spmv", ctypes.c_int),
                 ("gram_schmidt", ctypes.c_int), ("early_exit", ctypes.c_int), ("reorthogonalization", ctypes.c_int),
                 ("resid_recalc_freq", ctypes.c_int), ("save_history", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("device_resident", ctypes.c_int)]
+                ("device_resident", ctypes.c_int), ("preconditioner", ctypes.c_int),
+                ("use_precond_resid", ctypes.c_int)]
 
 
 class SolverResult(ctypes.Structure):

@@ -317,6 +317,7 @@ int hpb_csr_trsv_f32(hpb_handle_t h, hpb_trsv_t t, const float* val, float alpha
  * (common/csr/gpu_ilu0_preconditioner.h:62-72) does not factor; this one does.
  * Every row must store its diagonal entry. */
 int hpb_csr_ilu0_f64(hpb_handle_t h, hpb_trsv_t lower, double* val);
+int hpb_csr_ilu0_f32(hpb_handle_t h, hpb_trsv_t lower, float* val);
 
 /* ---- Krylov solvers on CSR matrices ---------------------------------------- */
 /* The solver templates (gmres.h:65-495, bicgstabl.h:68-275 of the reference's
@@ -325,6 +326,8 @@ int hpb_csr_ilu0_f64(hpb_handle_t h, hpb_trsv_t lower, double* val);
 #define HPB_SOLVER_GMRES 0
 #define HPB_SOLVER_BICGSTABL 1
 #define HPB_SOLVER_BICGSTAB 2 /* plain BiCGStab, numerical_algos/lin_solvers/bicgstab.h:134-253 */
+#define HPB_PRECOND_NONE 0
+#define HPB_PRECOND_ILU0 1
 typedef struct hpb_solver_params {
     int solver;              /* HPB_SOLVER_GMRES, HPB_SOLVER_BICGSTABL or HPB_SOLVER_BICGSTAB */
     int restart;             /* GMRES(m): m, 1 <= m < 256 (gmres.h:355-371) */
@@ -343,6 +346,10 @@ typedef struct hpb_solver_params {
                                 hipGraph per GMRES restart cycle / BiCGStab(L) cycle, no host round trip
                                 per dot); applies to CGS2 GMRES and BiCGStab(L <= 8), other settings run
                                 the host-driven templates.  Same recurrences and monitor rules. */
+    int preconditioner;      /* HPB_PRECOND_NONE or HPB_PRECOND_ILU0 (left preconditioning with an
+                                ILU(0) of A, csr/preconditioner.h; host-driven templates) */
+    int use_precond_resid;   /* with a preconditioner: the monitor tests ||P r|| against ||P b||
+                                (set_use_precond_resid, gmres.h:396-405); 0: unpreconditioned residual */
 } hpb_solver_params;
 typedef struct hpb_solver_result {
     int converged;           /* monitor converged (true residual for GMRES restarts) */
@@ -385,12 +392,17 @@ int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, dou
 int hpb_batch_info(hpb_batch_t b, int64_t* rows, int64_t* nnz, int* ntiles);
 /* Solve all systems to ||b_s - A_s x_s|| <= rel_tol ||b_s||.  use_graph: replay the
  * captured cycle (0: launch the same kernels directly); check_every: cycles between
- * host reads of the active count.  Per-system outputs (host, nsys entries, each
+ * host reads of the active count.  Every kernel of a cycle runs on a fixed grid that
+ * strides over the list of tiles of systems still active (rebuilt on the device at
+ * the end of each cycle), so converged systems cost nothing.  Per-system outputs (host, nsys entries, each
  * optional): iterations, true relative residual, converged flag; plus wall seconds
  * and cycles run. */
 int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int use_graph, int check_every,
                               int* out_iters, double* out_true_rel_resid, int* out_converged, double* out_seconds,
                               int* out_cycles);
+/* on != 0: Jacobi right preconditioning, A D^-1 (D x) = b with D = diag(A); the
+ * residual tested every cycle is still b - A x of the original system. */
+int hpb_batch_set_jacobi(hpb_batch_t b, int on);
 /* verbose != 0: print the count of active systems to stderr at every check
  * (the role of the reference monitor's log output, default_monitor.h:204-214) */
 int hpb_batch_set_verbose(hpb_batch_t b, int verbose);

@@ -128,3 +128,61 @@ def test_graph_and_direct_identical(handle):
         H.call("hpb_batch_destroy", b)
     assert np.array_equal(outs[0][0], outs[1][0])
     assert outs[0][1].tobytes() == outs[1][1].tobytes()
+
+
+def _solve(handle, b, ns, rel_tol=1e-10, max_iters=6000, use_graph=1, check_every=4):
+    it = np.zeros(ns, dtype=np.int32)
+    rr = np.zeros(ns)
+    cv = np.zeros(ns, dtype=np.int32)
+    H.call("hpb_batch_solve_bicgstab2", b, ctypes.c_double(rel_tol), max_iters, use_graph, check_every,
+           it.ctypes.data_as(ctypes.c_void_p), rr.ctypes.data_as(ctypes.c_void_p), cv.ctypes.data_as(ctypes.c_void_p),
+           None, None)
+    return it, rr, cv
+
+
+def test_jacobi_preconditioning(handle):
+    """Jacobi right preconditioning: same true residual test on the original
+    systems, fewer iterations on the heterogeneous coefficients, and x solves
+    A x = b (checked on the host with the original matrix)."""
+    r = np.random.default_rng(5)
+    sides = [int(v) for v in r.integers(20, 70, 12)]
+    ns = len(sides)
+    say("batch jacobi", sides)
+    b = make_batch(handle, sides)
+    it0, rr0, cv0 = _solve(handle, b, ns)
+    H.call("hpb_batch_set_jacobi", b, 1)
+    it1, rr1, cv1 = _solve(handle, b, ns)
+    assert cv0.all() and cv1.all()
+    assert rr1.max() <= 1e-10
+    assert it1.sum() < it0.sum(), (it0, it1)
+    rows = sum(m * m for m in sides)
+    x = np.empty(rows)
+    H.call("hpb_batch_solution", b, x.ctypes.data_as(ctypes.c_void_p), None)
+    rp, col, val, rhs = batch_host_csr(handle, b)
+    A = sp.csr_matrix((val, col, rp), shape=(rows, rows))
+    row0 = 0
+    for m in sides:
+        sl = slice(row0, row0 + m * m)
+        assert np.linalg.norm(rhs[sl] - A[sl, sl] @ x[sl]) <= 1.5e-10 * np.linalg.norm(rhs[sl])
+        row0 += m * m
+    # switching back re-records the cycle: the unpreconditioned iterates again
+    H.call("hpb_batch_set_jacobi", b, 0)
+    it2, rr2, cv2 = _solve(handle, b, ns)
+    assert np.array_equal(it2, it0) and np.array_equal(rr2, rr0)
+    H.call("hpb_batch_destroy", b)
+
+
+def test_tolerance_change_and_uneven_convergence(handle):
+    """A second solve with another tolerance re-records the captured cycle; with
+    very uneven sizes the small systems finish first and the rest still converge
+    (the tile list shrinks as systems finish)."""
+    sides = [2, 3, 90, 8, 61, 4]
+    ns = len(sides)
+    say("batch tolerance change", sides)
+    b = make_batch(handle, sides)
+    it_a, rr_a, cv_a = _solve(handle, b, ns, rel_tol=1e-6)
+    it_b, rr_b, cv_b = _solve(handle, b, ns, rel_tol=1e-11)
+    assert cv_a.all() and cv_b.all()
+    assert rr_a.max() <= 1e-6 and rr_b.max() <= 1e-11
+    assert (it_b >= it_a).all() and it_b.sum() > it_a.sum()
+    H.call("hpb_batch_destroy", b)
