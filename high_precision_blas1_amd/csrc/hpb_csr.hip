@@ -30,10 +30,11 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
                                                      const int* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ x, T alpha, T beta,
                                                      const T* __restrict__ b, T* __restrict__ y,
-                                                     const int* __restrict__ skip)
+                                                     const int* __restrict__ skip, const T* __restrict__ dalpha)
 {
     // MODE 0: y = alpha*A*x + beta*y (beta == 0: y not read); MODE 1: y = b - A*x
     if (skip != nullptr && *skip) return;   // solver graphs: flag set, launch does nothing
+    if (dalpha != nullptr) alpha = *dalpha;  // solver graphs: scale factor from device memory
     const int lane = threadIdx.x % LPR;
     const long row_stride = (long)gridDim.x * (SPMV_BLOCK / LPR);
     for (long r = ((long)blockIdx.x * SPMV_BLOCK + threadIdx.x) / LPR; r < rows; r += row_stride) {
@@ -68,10 +69,10 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
 
 template <typename T, bool COMP, int MODE, int LPR>
 static void spmv_launch_l(hpb_handle_t h, int grid, int rows, const int* rp, const int* col, const T* val,
-                          const T* x, T alpha, T beta, const T* b, T* y, const int* skip)
+                          const T* x, T alpha, T beta, const T* b, T* y, const int* skip, const T* dalpha)
 {
     hipLaunchKernelGGL((k_spmv<T, LPR, COMP, MODE>), dim3(grid), dim3(SPMV_BLOCK), 0, h->stream, rows, rp, col, val, x,
-                       alpha, beta, b, y, skip);
+                       alpha, beta, b, y, skip, dalpha);
 }
 
 static int pick_lpr(long rows, long nnz)
@@ -88,7 +89,8 @@ static int pick_lpr(long rows, long nnz)
 
 template <typename T, bool COMP, int MODE>
 static int spmv_entry(hpb_handle_t h, int rows, int cols, long nnz, const int* rp, const int* col, const T* val,
-                      const T* x, T alpha, T beta, const T* b, T* y, int lpr, const int* skip = nullptr)
+                      const T* x, T alpha, T beta, const T* b, T* y, int lpr, const int* skip = nullptr,
+                      const T* dalpha = nullptr)
 {
     HPB_CLEAR_STALE();
     HPB_REQUIRE(h, HPB_ERR_INVALID, "csr spmv: null handle");
@@ -105,13 +107,13 @@ static int spmv_entry(hpb_handle_t h, int rows, int cols, long nnz, const int* r
     if (grid > cap) grid = cap;
     if (grid < 1) grid = 1;
     switch (lpr) {
-    case 1: spmv_launch_l<T, COMP, MODE, 1>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip); break;
-    case 2: spmv_launch_l<T, COMP, MODE, 2>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip); break;
-    case 4: spmv_launch_l<T, COMP, MODE, 4>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip); break;
-    case 8: spmv_launch_l<T, COMP, MODE, 8>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip); break;
-    case 16: spmv_launch_l<T, COMP, MODE, 16>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip); break;
-    case 32: spmv_launch_l<T, COMP, MODE, 32>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip); break;
-    default: spmv_launch_l<T, COMP, MODE, 64>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip); break;
+    case 1: spmv_launch_l<T, COMP, MODE, 1>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha); break;
+    case 2: spmv_launch_l<T, COMP, MODE, 2>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha); break;
+    case 4: spmv_launch_l<T, COMP, MODE, 4>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha); break;
+    case 8: spmv_launch_l<T, COMP, MODE, 8>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha); break;
+    case 16: spmv_launch_l<T, COMP, MODE, 16>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha); break;
+    case 32: spmv_launch_l<T, COMP, MODE, 32>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha); break;
+    default: spmv_launch_l<T, COMP, MODE, 64>(h, (int)grid, rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha); break;
     }
     HPB_LAUNCH_CHECK();
     return HPB_OK;
@@ -120,19 +122,21 @@ static int spmv_entry(hpb_handle_t h, int rows, int cols, long nnz, const int* r
 // Internal launcher for the solver graphs: MODE 0 y = A x, MODE 1 y = b - A x.
 template <typename T>
 int csr_apply(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* col, const T* val, const T* x,
-              const T* b, T* y, bool comp, const int* skip)
+              const T* b, T* y, bool comp, const int* skip, const T* dev_alpha)
 {
     if (b) {
         return comp ? spmv_entry<T, true, 1>(h, rows, rows, nnz, rp, col, val, x, T(1), T(0), b, y, 0, skip)
                     : spmv_entry<T, false, 1>(h, rows, rows, nnz, rp, col, val, x, T(1), T(0), b, y, 0, skip);
     }
-    return comp ? spmv_entry<T, true, 0>(h, rows, rows, nnz, rp, col, val, x, T(1), T(0), nullptr, y, 0, skip)
-                : spmv_entry<T, false, 0>(h, rows, rows, nnz, rp, col, val, x, T(1), T(0), nullptr, y, 0, skip);
+    return comp ? spmv_entry<T, true, 0>(h, rows, rows, nnz, rp, col, val, x, T(1), T(0), nullptr, y, 0, skip,
+                                         dev_alpha)
+                : spmv_entry<T, false, 0>(h, rows, rows, nnz, rp, col, val, x, T(1), T(0), nullptr, y, 0, skip,
+                                          dev_alpha);
 }
 template int csr_apply<double>(hpb_handle_t, int, int64_t, const int*, const int*, const double*, const double*,
-                               const double*, double*, bool, const int*);
+                               const double*, double*, bool, const int*, const double*);
 template int csr_apply<float>(hpb_handle_t, int, int64_t, const int*, const int*, const float*, const float*,
-                              const float*, float*, bool, const int*);
+                              const float*, float*, bool, const int*, const float*);
 
 // ---------------------------------------------------------------------------
 // structured test matrices

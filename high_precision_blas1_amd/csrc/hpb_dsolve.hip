@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "hpb_device.hpp"
@@ -41,6 +42,7 @@ constexpr int MAXL = 8;        // BiCGStab(L) basis in the device path
 
 struct dstate {
     double tol, bnorm, rnorm;
+    double rlast;   // reliable updates: FP64 true residual norm at the last update
     double rho, alpha, omega;
     int iters, max_iters, min_iters, cycles;
     int done, stop, converged, breakdown, invalid;
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(DB) void k_fill(size_t n, T* __restrict__ v, T a)
 
 // restart: beta = ||r||, monitor check on the true residual, g = beta e_1
 template <typename T>
-__global__ void k_gm_start(dstate* st, const T* n4, double* g, int m, T* scal, double* hist)
+__global__ void k_gm_start(dstate* st, const T* n4, double* g, int m, T* sig, double* hist)
 {
     if (st->done) {
         st->stop = 1;
@@ -141,21 +143,7 @@ __global__ void k_gm_start(dstate* st, const T* n4, double* g, int m, T* scal, d
     st->cycles++;
     g[0] = beta;
     for (int i = 1; i <= m; ++i) g[i] = 0.0;
-    *scal = (T)(1.0 / beta);
-}
-
-// Hessenberg column j from a multi-dot pass; coef = -h for the multi-update
-template <typename T>
-__global__ __launch_bounds__(DB) void k_gm_coef(int j, const T* __restrict__ d4, double* __restrict__ H, int ldh,
-                                                T* __restrict__ coef, int pass, const int* __restrict__ stop)
-{
-    if (*stop) return;
-    for (int i = threadIdx.x; i <= j; i += DB) {
-        const double h = dval(d4 + 4 * i);
-        if (pass == 0) H[(size_t)j * ldh + i] = h;
-        else H[(size_t)j * ldh + i] += h;
-        coef[i] = (T)(-h);
-    }
+    sig[0] = (T)(1.0 / beta);
 }
 
 __device__ __forceinline__ void apply_rot(double& dx, double& dy, double cs, double sn)
@@ -165,14 +153,14 @@ __device__ __forceinline__ void apply_rot(double& dx, double& dy, double cs, dou
     dx = t;
 }
 
-// new column j: previous rotations, new rotation, residual estimate (gmres.h:217-274)
+// New Hessenberg column j with h_{j+1,j} = hn: previous rotations, new rotation,
+// residual estimate (gmres.h:217-274); sig[j+1] = 1 / hn scales the stored basis
+// vector (V holds unnormalised vectors; the basis is sig_i V_i).
 template <typename T>
-__global__ void k_gm_givens(int j, int m, const T* n4, double* H, int ldh, double* cs, double* sn, double* g,
-                            T* scal, dstate* st, double* hist)
+__device__ void givens_step(int j, int m, double hn, double* H, int ldh, double* cs, double* sn, double* g, T* sig,
+                            dstate* st)
 {
-    if (st->stop) return;
     double* Hj = H + (size_t)j * ldh;
-    const double hn = sqrt(dval(n4));
     Hj[j + 1] = hn;
     for (int k = 0; k < j; ++k) apply_rot(Hj[k], Hj[k + 1], cs[k], sn[k]);
     double c, s;
@@ -198,14 +186,157 @@ __global__ void k_gm_givens(int j, int m, const T* n4, double* H, int ldh, doubl
     st->cycle_len = j + 1;
     const double est = fabs(g[j + 1]);
     st->rnorm = est;
-    *scal = hn != 0.0 ? (T)(1.0 / hn) : T(0);
+    sig[j + 1] = hn != 0.0 ? (T)(1.0 / hn) : T(0);
     if (hn == 0.0 || !finite(est) || (st->early_exit && est <= st->tol) || j + 1 >= m || st->iters >= st->max_iters)
         st->stop = 1;
 }
 
+template <typename T>
+struct basis_pack {
+    const T* p[MAXB];
+};
+
+template <typename T> struct ubits { using type = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned>; };
+template <typename T>
+__device__ __forceinline__ void st_ag(T* p, T v)
+{
+    using U = typename ubits<T>::type;
+    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_ag(const T* p)
+{
+    using U = typename ubits<T>::type;
+    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ __forceinline__ bool last_block(unsigned* ticket, unsigned total)
+{
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        am_last = (prev == total - 1);
+    }
+    __syncthreads();
+    const bool last = am_last != 0;
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return last;
+}
+
+// One Gram-Schmidt projection pass for column j: h_i = (sig_i V_i, w), i <= j,
+// compensated or plain, KC columns per blockIdx.y chunk.  The block that takes
+// the last ticket merges each column's partials in block order and writes
+// H(i, j) (pass 0) or adds to it (pass 1), and coef_i = -h_i sig_i for the update.
+template <typename T, int KC, bool COMP>
+__global__ __launch_bounds__(DB) void k_gm_dot(size_t n, int k, basis_pack<T> V, const T* __restrict__ w,
+                                               const T* __restrict__ sig, T* __restrict__ part,
+                                               unsigned* __restrict__ ticket, double* __restrict__ Hj,
+                                               T* __restrict__ coef, int pass, const int* __restrict__ stop)
+{
+    if (*stop) return;
+    const unsigned G = gridDim.x;
+    const int j0 = blockIdx.y * KC;
+    const int kc = (k - j0) < KC ? (k - j0) : KC;
+    T s[KC], c[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) s[q] = c[q] = T(0);
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)G * DB) {
+        const T a = w[i];
+#pragma unroll
+        for (int q = 0; q < KC; ++q)
+            if (q < kc) {
+                const T v = V.p[j0 + q][i];
+                if (COMP) dev::dot2_step<T>(s[q], c[q], v, a);
+                else s[q] = dev::fma_rn<T>(v, a, s[q]);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < KC; ++q)
+        if (q < kc) {
+            T ss = s[q], cc = c[q];
+            dev::block_reduce<T, COMP, DB>(ss, cc);
+            if (threadIdx.x == 0) {
+                st_ag<T>(&part[2 * ((size_t)(j0 + q) * G + blockIdx.x)], ss);
+                st_ag<T>(&part[2 * ((size_t)(j0 + q) * G + blockIdx.x) + 1], cc);
+            }
+        }
+    if (!last_block(ticket, G * gridDim.y)) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int col = wid; col < k; col += DB / 64) {
+        T fs = T(0), fc = T(0);
+        for (unsigned b = lane; b < G; b += 64) {
+            const T ps = ld_ag<T>(&part[2 * ((size_t)col * G + b)]);
+            const T pc = ld_ag<T>(&part[2 * ((size_t)col * G + b) + 1]);
+            if (COMP) dev::merge<T>(fs, fc, ps, pc);
+            else fs = fs + ps;
+        }
+        dev::wave_reduce<T, COMP>(fs, fc);
+        if (lane == 0) {
+            const double sg = (double)sig[col];
+            const double h = ((double)fs + (double)fc) * sg;
+            if (pass == 0) Hj[col] = h;
+            else Hj[col] += h;
+            coef[col] = (T)(-h * sg);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// w += sum_{i<k} coef_i V_i (Dot2 over the terms of each element when COMP);
+// NORM: also ||w||^2 (compensated), and the last block runs the Givens step for
+// column j with h_{j+1,j} = ||w||.
+template <typename T, bool COMP, bool NORM>
+__global__ __launch_bounds__(DB) void k_gm_upd(size_t n, int k, basis_pack<T> V, const T* __restrict__ coef,
+                                               T* __restrict__ w, T* __restrict__ part, unsigned* __restrict__ ticket,
+                                               int j, int m, double* __restrict__ H, int ldh, double* __restrict__ cs,
+                                               double* __restrict__ sn, double* __restrict__ g, T* __restrict__ sig,
+                                               dstate* __restrict__ st)
+{
+    if (st->stop) return;
+    __shared__ T sc[MAXB];
+    for (int q = threadIdx.x; q < k; q += DB) sc[q] = coef[q];
+    __syncthreads();
+    T ns = T(0), nc = T(0);
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        T s = w[i], c = T(0);
+        for (int q = 0; q < k; ++q) {
+            if (COMP) dev::dot2_step<T>(s, c, sc[q], V.p[q][i]);
+            else s = dev::fma_rn<T>(sc[q], V.p[q][i], s);
+        }
+        const T r = COMP ? s + c : s;
+        w[i] = r;
+        if (NORM) {
+            if (COMP) dev::dot2_step<T>(ns, nc, r, r);
+            else ns = dev::fma_rn<T>(r, r, ns);
+        }
+    }
+    if (!NORM) return;
+    dev::block_reduce<T, COMP, DB>(ns, nc);
+    const unsigned G = gridDim.x;
+    if (threadIdx.x == 0) {
+        st_ag<T>(&part[2 * blockIdx.x], ns);
+        st_ag<T>(&part[2 * blockIdx.x + 1], nc);
+    }
+    if (!last_block(ticket, G)) return;
+    T fs = T(0), fc = T(0);
+    for (unsigned b = threadIdx.x; b < G; b += DB) {
+        const T ps = ld_ag<T>(&part[2 * b]);
+        const T pc = ld_ag<T>(&part[2 * b + 1]);
+        if (COMP) dev::merge<T>(fs, fc, ps, pc);
+        else fs = fs + ps;
+    }
+    dev::block_reduce<T, COMP, DB>(fs, fc);
+    if (threadIdx.x == 0) {
+        givens_step<T>(j, m, sqrt((double)fs + (double)fc), H, ldh, cs, sn, g, sig, st);
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // y = R^{-1} g over the steps of this cycle (gmres.h:204-215); y[k..m) = 0
 template <typename T>
-__global__ void k_gm_backsub(int m, const double* H, int ldh, double* g, T* y, const dstate* st)
+__global__ void k_gm_backsub(int m, const double* H, int ldh, double* g, const T* sig, T* y, const dstate* st)
 {
     if (st->done) return;
     const int k = st->cycle_len;
@@ -213,13 +344,9 @@ __global__ void k_gm_backsub(int m, const double* H, int ldh, double* g, T* y, c
         g[j] /= H[(size_t)j * ldh + j];
         for (int i = j - 1; i >= 0; --i) g[i] -= H[(size_t)j * ldh + i] * g[j];
     }
-    for (int j = 0; j < m; ++j) y[j] = j < k ? (T)g[j] : T(0);
+    // x += sum_j g_j (sig_j V_j)
+    for (int j = 0; j < m; ++j) y[j] = j < k ? (T)(g[j] * (double)sig[j]) : T(0);
 }
-
-template <typename T>
-struct basis_pack {
-    const T* p[MAXB];
-};
 
 // x += sum_{j < cycle_len} y_j V_j (compensated: Dot2 over the terms of each element)
 template <typename T, bool COMP>
@@ -261,15 +388,21 @@ __global__ void k_bs_init(dstate* st, const T* n4, T* scal, double* hist)
 }
 
 // rho1 = (r_j, rt); beta = alpha rho1 / rho (bicgstabl.h:199-206)
-template <typename T>
-__global__ void k_bs_beta(dstate* st, const T* d4, T* coef, int first)
+__device__ __forceinline__ void breakdown_stop(dstate* st)
 {
-    if (st->done) return;
+    st->breakdown = 1;
+    st->done = 1;
+    st->stop = 1;
+}
+
+template <typename T>
+__global__ void k_bs_beta(dstate* st, const T* d4, T* coef, int first, const int* skip)
+{
+    if (*skip) return;
     double rho = st->rho;
     if (first) rho = -st->omega * rho;
     if (rho == 0.0 || !finite(rho)) {
-        st->breakdown = 1;
-        st->done = 1;
+        breakdown_stop(st);
         return;
     }
     const double rho1 = dval(d4);
@@ -280,13 +413,12 @@ __global__ void k_bs_beta(dstate* st, const T* d4, T* coef, int first)
 
 // alpha = rho / (u_{j+1}, rt)
 template <typename T>
-__global__ void k_bs_alpha(dstate* st, const T* d4, T* coef)
+__global__ void k_bs_alpha(dstate* st, const T* d4, T* coef, const int* skip)
 {
-    if (st->done) return;
+    if (*skip) return;
     const double sg = dval(d4);
     if (sg == 0.0 || !finite(sg)) {
-        st->breakdown = 1;
-        st->done = 1;
+        breakdown_stop(st);
         return;
     }
     st->alpha = st->rho / sg;
@@ -320,9 +452,9 @@ __global__ __launch_bounds__(DB) void k_bs_upd_r(size_t n, int j, bs_pack<T> P, 
 // MR part: gram[k] holds (r_i, r_k), i = 0..k, as 4-word (s, c, ...) entries.
 // Solves G g = c by Cholesky in FP64; coef[2 + j] = g_{j+1}.
 template <typename T>
-__global__ void k_bs_mr(dstate* st, int L, const T* gram, int gstride, T* coef)
+__global__ void k_bs_mr(dstate* st, int L, const T* gram, int gstride, T* coef, const int* skip)
 {
-    if (st->done) return;
+    if (*skip) return;
     double G[MAXL][MAXL], c[MAXL], y[MAXL];
     for (int k = 1; k <= L; ++k) {
         const T* gk = gram + (size_t)k * gstride;   // entries (r_i, r_k), i = 0..k
@@ -338,8 +470,7 @@ __global__ void k_bs_mr(dstate* st, int L, const T* gram, int gstride, T* coef)
         double d = G[j][j];
         for (int k = 0; k < j; ++k) d -= G[k][j] * G[k][j];
         if (!(d > 0.0) || !finite(d)) {
-            st->breakdown = 1;
-            st->done = 1;
+            breakdown_stop(st);
             return;
         }
         d = sqrt(d);
@@ -413,6 +544,71 @@ __global__ void k_bs_check(dstate* st, const T* n4, double* hist)
     st->iters += st->basis;
     st->cycles++;
     monitor_check(st, hist, sqrt(dval(n4)));
+}
+
+// ---- reliable updates (FP32 iterations, FP64 solution and true residual)
+
+// after a cycle: the FP32 recursive residual decides whether the FP64 solution is
+// updated now (reduction by delta since the last update, tolerance reached, or the
+// iteration cap); stop pauses the cycle graph until the host ran the update
+template <typename T>
+__global__ void k_rel_check(dstate* st, const T* n4, double delta)
+{
+    if (st->stop) return;
+    st->iters += st->basis;
+    st->cycles++;
+    const double rn = sqrt(dval(n4));
+    st->rnorm = rn;
+    if (!finite(rn)) {
+        st->invalid = 1;
+        st->done = 1;
+        st->stop = 1;
+        return;
+    }
+    if (rn <= delta * st->rlast || rn <= st->tol || st->iters >= st->max_iters) st->stop = 1;
+}
+
+// x64 += x32, x32 = 0
+__global__ __launch_bounds__(DB) void k_rel_flush(size_t n, float* __restrict__ x32, double* __restrict__ x64,
+                                                  const int* __restrict__ skip)
+{
+    if (*skip) return;
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        x64[i] += (double)x32[i];
+        x32[i] = 0.0f;
+    }
+}
+
+// the FP64 true residual norm: monitor check, new reference for delta, resume
+__global__ void k_rel_after(dstate* st, const double* n4, double* hist, int first)
+{
+    if (st->done) return;
+    const double rn = sqrt(dval(n4));
+    st->rlast = rn;
+    monitor_check(st, hist, rn);
+    if (first) {
+        st->rho = 1.0;
+        st->alpha = 0.0;
+        st->omega = 1.0;
+    }
+    st->stop = st->done;
+}
+
+// r32 = (float) r64; on the first update also r~ = r32 / ||r32|| and u_0 = 0
+__global__ __launch_bounds__(DB) void k_rel_convert(size_t n, const double* __restrict__ r64, float* __restrict__ r32,
+                                                    float* __restrict__ rt, float* __restrict__ u0,
+                                                    const dstate* __restrict__ st, int first)
+{
+    if (st->done) return;
+    const double inv = st->rlast > 0.0 ? 1.0 / st->rlast : 1.0;
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        const float v = (float)r64[i];
+        r32[i] = v;
+        if (first) {
+            rt[i] = (float)(r64[i] * inv);
+            u0[i] = 0.0f;
+        }
+    }
 }
 
 // ---------------------------------------------------------------- host side
@@ -543,20 +739,39 @@ int init_state(hpb_handle_t h, dstate* d, const hpb_solver_params& p, int basis,
 // One solver instance: buffers and the captured cycle graph live as long as the
 // object, so repeated solves with the same operands (the inner solves of
 // iterative refinement) replay the graph without allocating or capturing again.
+// FP64 side of the reliable-update mode: the FP32 iteration runs on P (FP32
+// values, right-hand side slot unused, x = the FP32 increment), the FP64 solution
+// x64 and true residual b64 - A64 x64 are updated every time the FP32 recursive
+// residual has dropped by delta.
+struct rel64 {
+    const double* val;
+    const double* b;
+    double* x;
+    double delta;
+};
+
 template <typename T>
 class dsolver : public dsolver_base {
 public:
     problem<T> P{};
     hpb_solver_params prm{};
     int hcap = 0;
+    bool reliable = false;
+    rel64 R{};
 
-    int setup(const problem<T>& pr, const hpb_solver_params& p, int hist_cap)
+    int setup(const problem<T>& pr, const hpb_solver_params& p, int hist_cap, const rel64* rel = nullptr)
     {
         P = pr;
         prm = p;
         hcap = hist_cap;
+        reliable = rel != nullptr;
+        if (rel) R = *rel;
         DS_CHECK(dst.alloc(1));
         if (hcap) DS_CHECK(dhist.alloc(2 * (size_t)hcap));
+        if (reliable) {
+            DS_CHECK(r64.alloc((size_t)P.rows));
+            DS_CHECK(n4d.alloc(4));
+        }
         return p.solver == HPB_SOLVER_GMRES ? setup_gmres() : setup_bicgstabl();
     }
     bool matches(const problem<T>& pr, const hpb_solver_params& p, int hist_cap) const
@@ -579,6 +794,7 @@ public:
         prm.early_exit = p.early_exit;
         prm.resid_recalc_freq = p.resid_recalc_freq;
         DS_CHECK(init_state<T>(P.h, dst.p, prm, prm.solver == HPB_SOLVER_GMRES ? 1 : prm.basis, hcap));
+        if (reliable) return run_reliable(out);
         return prm.solver == HPB_SOLVER_GMRES ? run_gmres(out) : run_bicgstabl(out);
     }
     const double* history() const { return dhist.p; }
@@ -591,6 +807,7 @@ private:
     std::vector<T*> V;
     dbuf<double> H, cs, sn, g;
     dbuf<T> d4, n4, coef, y, scal, rt, da, db, gram;
+    dbuf<double> r64, n4d;
     int gstride = 0;
 
     double* hist() { return hcap ? dhist.p : nullptr; }
@@ -613,38 +830,63 @@ private:
         DS_CHECK(cs.alloc(m));
         DS_CHECK(sn.alloc(m));
         DS_CHECK(g.alloc(m + 1));
-        DS_CHECK(d4.alloc(4 * (size_t)(m + 1)));
         DS_CHECK(n4.alloc(4));
         DS_CHECK(coef.alloc(m + 1));
         DS_CHECK(y.alloc(m));
-        DS_CHECK(scal.alloc(1));
+        DS_CHECK(scal.alloc(m + 1));   // sig_i: the basis is sig_i V_i
         dstate* S = dst.p;
         const int* done = &S->done;
         const int* stop = &S->stop;
         basis_pack<T> bp{};
-        for (int j = 0; j < m; ++j) bp.p[j] = V[j];
+        for (int j = 0; j <= m; ++j) bp.p[j] = V[j];
         double* hp = hist();
+        T* sig = scal.p;
+        T* part = reinterpret_cast<T*>(h->part);
+        unsigned* tk = h->ticket + 3;
+        const size_t cus = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
+        const size_t part_pairs = h->part_bytes / (2 * sizeof(T));
+        const size_t gfull = (n + DB - 1) / DB;
+        const unsigned gupd = (unsigned)std::max<size_t>(1, std::min(gfull, 8 * cus));
+        constexpr int KC = 8;
         HPB_HIP(hipStreamSynchronize(st));
         HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = [&]() -> int {
             DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, P.x, P.b, V[0], cspmv, done));
             DS_CHECK(nrm2_async<T>(h, n, V[0], n4.p, comp, done));
-            hipLaunchKernelGGL((k_gm_start<T>), dim3(1), dim3(1), 0, st, S, n4.p, g.p, m, scal.p, hp);
-            hipLaunchKernelGGL((k_scale<T>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, V[0], scal.p, stop);
+            hipLaunchKernelGGL((k_gm_start<T>), dim3(1), dim3(1), 0, st, S, n4.p, g.p, m, sig, hp);
             for (int j = 0; j < m; ++j) {
-                DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, V[j], nullptr, V[j + 1], cspmv, stop));
+                const int k = j + 1;
+                // w = A (sig_j V_j), stored unnormalised in V[j+1]
+                DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, V[j], nullptr, V[j + 1], cspmv, stop,
+                                      sig + j));
+                const unsigned nch = (unsigned)((k + KC - 1) / KC);
+                size_t gd = std::min(gfull, std::max<size_t>(cus / 2, (8 * cus) / nch));
+                gd = std::max<size_t>(1, std::min(gd, part_pairs / (size_t)k));
+                double* Hj = H.p + (size_t)j * ldh;
                 for (int pass = 0; pass < 2; ++pass) {
-                    DS_CHECK(multidot_chunks<T>(h, j + 1, V.data(), n, V[j + 1], comp, d4.p, stop));
-                    hipLaunchKernelGGL((k_gm_coef<T>), dim3(1), dim3(DB), 0, st, j, d4.p, H.p, ldh, coef.p, pass,
-                                       stop);
-                    DS_CHECK(multiaxpy_chunks<T>(h, j + 1, coef.p, V.data(), n, V[j + 1], comp, stop));
+                    if (comp)
+                        hipLaunchKernelGGL((k_gm_dot<T, KC, true>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, k, bp,
+                                           V[j + 1], sig, part, tk, Hj, coef.p, pass, stop);
+                    else
+                        hipLaunchKernelGGL((k_gm_dot<T, KC, false>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, k,
+                                           bp, V[j + 1], sig, part, tk, Hj, coef.p, pass, stop);
+                    auto upd = [&](auto compc, auto normc) {
+                        hipLaunchKernelGGL((k_gm_upd<T, decltype(compc)::value, decltype(normc)::value>), dim3(gupd),
+                                           dim3(DB), 0, st, n, k, bp, coef.p, V[j + 1], part, tk, j, m, H.p, ldh,
+                                           cs.p, sn.p, g.p, sig, S);
+                    };
+                    using BT = std::true_type;
+                    using BF = std::false_type;
+                    if (comp) {
+                        if (pass) upd(BT{}, BT{});
+                        else upd(BT{}, BF{});
+                    } else {
+                        if (pass) upd(BF{}, BT{});
+                        else upd(BF{}, BF{});
+                    }
                 }
-                DS_CHECK(nrm2_async<T>(h, n, V[j + 1], n4.p, comp, stop));
-                hipLaunchKernelGGL((k_gm_givens<T>), dim3(1), dim3(1), 0, st, j, m, n4.p, H.p, ldh, cs.p, sn.p, g.p,
-                                   scal.p, S, hp);
-                hipLaunchKernelGGL((k_scale<T>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, V[j + 1], scal.p, stop);
             }
-            hipLaunchKernelGGL((k_gm_backsub<T>), dim3(1), dim3(1), 0, st, m, H.p, ldh, g.p, y.p, S);
+            hipLaunchKernelGGL((k_gm_backsub<T>), dim3(1), dim3(1), 0, st, m, H.p, ldh, g.p, sig, y.p, S);
             if (comp)
                 hipLaunchKernelGGL((k_gm_xupd<T, true>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, bp, y.p, P.x, S);
             else
@@ -700,7 +942,8 @@ private:
         DS_CHECK(coef.alloc(2 + MAXL));
         DS_CHECK(scal.alloc(1));
         dstate* S = dst.p;
-        const int* done = &S->done;
+        // reliable mode pauses the cycle with stop while the host updates x64
+        const int* done = reliable ? &S->stop : &S->done;
         const unsigned vg = vgrid(h, n);
         double* hp = hist();
         HPB_HIP(hipStreamSynchronize(st));
@@ -708,23 +951,28 @@ private:
         int rc = [&]() -> int {
             for (int j = 0; j < L; ++j) {
                 DS_CHECK(dot2_async<T>(h, n, V[j], rt.p, da.p, comp, done));
-                hipLaunchKernelGGL((k_bs_beta<T>), dim3(1), dim3(1), 0, st, S, da.p, coef.p, j == 0 ? 1 : 0);
+                hipLaunchKernelGGL((k_bs_beta<T>), dim3(1), dim3(1), 0, st, S, da.p, coef.p, j == 0 ? 1 : 0, done);
                 hipLaunchKernelGGL((k_bs_upd_u<T>), dim3(vg), dim3(DB), 0, st, n, j, bp, coef.p, done);
                 DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, Ub[j].p, nullptr, Ub[j + 1].p, cspmv,
                                       done));
                 DS_CHECK(dot2_async<T>(h, n, Ub[j + 1].p, rt.p, db.p, comp, done));
-                hipLaunchKernelGGL((k_bs_alpha<T>), dim3(1), dim3(1), 0, st, S, db.p, coef.p);
+                hipLaunchKernelGGL((k_bs_alpha<T>), dim3(1), dim3(1), 0, st, S, db.p, coef.p, done);
                 hipLaunchKernelGGL((k_bs_upd_r<T>), dim3(vg), dim3(DB), 0, st, n, j, bp, P.x, coef.p, done);
                 DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, V[j], nullptr, V[j + 1], cspmv, done));
             }
             for (int k = 1; k <= L; ++k)
                 DS_CHECK(multidot_async<T>(h, k + 1, (const T* const*)V.data(), n, V[k], comp,
                                            gram.p + (size_t)k * gstride, done));
-            hipLaunchKernelGGL((k_bs_mr<T>), dim3(1), dim3(1), 0, st, S, L, gram.p, gstride, coef.p);
+            hipLaunchKernelGGL((k_bs_mr<T>), dim3(1), dim3(1), 0, st, S, L, gram.p, gstride, coef.p, done);
             if (comp)
                 hipLaunchKernelGGL((k_bs_upd_mr<T, true>), dim3(vg), dim3(DB), 0, st, n, L, bp, P.x, coef.p, done);
             else
                 hipLaunchKernelGGL((k_bs_upd_mr<T, false>), dim3(vg), dim3(DB), 0, st, n, L, bp, P.x, coef.p, done);
+            if (reliable) {
+                DS_CHECK(nrm2_async<T>(h, n, V[0], n4.p, comp, done));
+                hipLaunchKernelGGL((k_rel_check<T>), dim3(1), dim3(1), 0, st, S, n4.p, R.delta);
+                return HPB_OK;
+            }
             hipLaunchKernelGGL(k_bs_recalc_flag, dim3(1), dim3(1), 0, st, S);
             DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, P.x, P.b, V[0], cspmv, &S->skip_recalc));
             DS_CHECK(nrm2_async<T>(h, n, V[0], n4.p, comp, done));
@@ -763,6 +1011,61 @@ private:
         }
         return HPB_OK;
     }
+
+    // FP64 update: x64 += x32, x32 = 0, r64 = b64 - A64 x64 (compensated rows),
+    // monitor check on ||r64||, r32 = (float) r64 (and on the first call r~, u_0)
+    int rel_update(int first)
+    {
+        hpb_handle_t h = P.h;
+        hipStream_t st = h->stream;
+        const size_t n = (size_t)P.rows;
+        const unsigned vg = vgrid(h, n);
+        dstate* S = dst.p;
+        if (!first)
+            hipLaunchKernelGGL(k_rel_flush, dim3(vg), dim3(DB), 0, st, n, (float*)P.x, R.x, &S->done);
+        DS_CHECK(csr_apply<double>(h, P.rows, P.nnz, P.rp, P.ci, R.val, R.x, R.b, r64.p, true, &S->done));
+        DS_CHECK(nrm2_async<double>(h, n, r64.p, n4d.p, true, &S->done));
+        hipLaunchKernelGGL(k_rel_after, dim3(1), dim3(1), 0, st, S, n4d.p, hist(), first);
+        hipLaunchKernelGGL(k_rel_convert, dim3(vg), dim3(DB), 0, st, n, r64.p, (float*)V[0], (float*)rt.p,
+                           (float*)Ub[0].p, S, first);
+        HPB_LAUNCH_CHECK();
+        return HPB_OK;
+    }
+
+    int run_reliable(dstate* out)
+    {
+        hpb_handle_t h = P.h;
+        hipStream_t st = h->stream;
+        const size_t n = (size_t)P.rows;
+        const int L = prm.basis;
+        dstate* S = dst.p;
+        DS_CHECK(nrm2_async<double>(h, n, R.b, n4d.p, true, nullptr));
+        hipLaunchKernelGGL((k_init<double>), dim3(1), dim3(1), 0, st, S, n4d.p, prm.rel_tol, prm.abs_tol);
+        hipLaunchKernelGGL((k_fill<T>), dim3(vgrid(h, n)), dim3(DB), 0, st, n, P.x, T(0));
+        DS_CHECK(rel_update(1));
+        DS_CHECK(read_state<T>(h, S, out));
+        const int max_cycles = prm.max_iters / L + 2;
+        const int every = 4;
+        int cycles = 0;
+        while (!out->done && cycles < max_cycles) {
+            for (int k = 0; k < every; ++k) HPB_HIP(hipGraphLaunch(gg.e, st));
+            cycles += every;
+            DS_CHECK(read_state<T>(h, S, out));
+            if (out->done) break;
+            if (out->stop) {
+                DS_CHECK(rel_update(0));
+                DS_CHECK(read_state<T>(h, S, out));
+                updates++;
+            }
+        }
+        // keep the progress of an unfinished run (x32 holds the increment since the last update)
+        hipLaunchKernelGGL(k_rel_flush, dim3(vgrid(h, n)), dim3(DB), 0, st, n, (float*)P.x, R.x, &S->invalid);
+        HPB_LAUNCH_CHECK();
+        return HPB_OK;
+    }
+
+public:
+    int updates = 0;
 };
 
 }  // namespace
@@ -819,6 +1122,37 @@ int dsolve(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, 
 }
 
 void dsolve_release(dsolver_base* cache) { delete cache; }
+
+// FP32 BiCGStab(L) with reliable updates of an FP64 solution: val32 / val64 share
+// rp / ci; b64, x64 FP64; r32, x32 FP32 work vectors of the caller.  The inner
+// rel_tol is the reduction delta of the FP32 recursive residual that triggers an
+// FP64 update; rel_tol is the final tolerance on ||b - A x|| / ||b||.
+int dsolve_reliable(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, const float* val32,
+                    const double* val64, const double* b64, double* x64, float* r32, float* x32,
+                    const hpb_solver_params& inner, double rel_tol, hpb_solver_result* res, int* updates)
+{
+    HPB_REQUIRE(inner.solver == HPB_SOLVER_BICGSTABL && inner.basis >= 1 && inner.basis <= MAXL, HPB_ERR_UNSUPPORTED,
+                "reliable updates: BiCGStab(L <= 8) inner solver only");
+    HPB_REQUIRE(inner.rel_tol > 0.0 && inner.rel_tol < 1.0, HPB_ERR_INVALID,
+                "reliable updates: the inner rel_tol (update threshold) must be in (0, 1)");
+    problem<float> P{h, rows, nnz, rp, ci, val32, r32, x32};
+    hpb_solver_params p = inner;
+    const double delta = p.rel_tol;
+    p.rel_tol = rel_tol;   // the monitor tests the FP64 true residual against the final tolerance
+    rel64 R{val64, b64, x64, delta};
+    dsolver<float> ds;
+    DS_CHECK(ds.setup(P, p, 0, &R));
+    dstate out{};
+    DS_CHECK(ds.run(p, &out));
+    res->iters = out.iters;
+    res->converged = out.converged;
+    res->breakdown = out.breakdown;
+    res->resid_norm = out.rlast;
+    res->rhs_norm = out.bnorm;
+    res->history_len = 0;
+    if (updates) *updates = ds.updates;
+    return HPB_OK;
+}
 
 template int dsolve<double>(hpb_handle_t, int, int64_t, const int*, const int*, const double*, const double*,
                             double*, const hpb_solver_params&, hpb_solver_result*, double*, int, dsolver_base**);

@@ -68,10 +68,14 @@ int dsolve(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, 
            const hpb_solver_params& p, hpb_solver_result* res, double* history, int history_cap,
            dsolver_base** cache = nullptr);
 void dsolve_release(dsolver_base* cache);
-// b == null: y = A x; else y = b - A x
+// FP32 BiCGStab(L) iterations with reliable updates of an FP64 solution
+int dsolve_reliable(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, const float* val32,
+                    const double* val64, const double* b64, double* x64, float* r32, float* x32,
+                    const hpb_solver_params& inner, double rel_tol, hpb_solver_result* res, int* updates);
+// b == null: y = a A x with a = *dev_alpha (1 when dev_alpha is null); else y = b - A x
 template <typename T>
 int csr_apply(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* col, const T* val, const T* x,
-              const T* b, T* y, bool comp, const int* skip);
+              const T* b, T* y, bool comp, const int* skip, const T* dev_alpha = nullptr);
 
 }  // namespace hpb
 

@@ -220,8 +220,13 @@ static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, co
     const int nch = (k + KC - 1) / KC;
     const size_t W = vec ? dev::vec16<T>::W : 1;
     const size_t nv = n / W;
+    // about 8 blocks per CU in total over the column chunks: one chunk alone
+    // needs several blocks per CU to keep enough loads in flight
     size_t g = (nv + 255) / 256;
-    const size_t cap = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
+    const size_t cus = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
+    size_t cap = (8 * cus) / (size_t)nch;
+    if (cap < cus / 2) cap = cus / 2;
+    if (cap * (size_t)k > (size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) cap = ((size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) / k;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
     T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);

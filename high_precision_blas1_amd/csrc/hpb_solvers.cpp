@@ -168,6 +168,7 @@ void hpb_solver_params_default(hpb_solver_params* p)
     p->device_resident = 1;
     p->preconditioner = HPB_PRECOND_NONE;
     p->use_precond_resid = 1;
+    p->refinement = 1;
 }
 
 int hpb_csr_solve_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
@@ -280,6 +281,24 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
         int outer = 0, inner_total = 0;
         bool conv = false;
         double rn = 0.0;
+        if (inner->solver == HPB_SOLVER_BICGSTABL && inner->refinement == 1 && use_device(*inner)) {
+            // reliable updates: one FP32 BiCGStab(L) iteration feeding an FP64 solution
+            hpb_solver_result ir{};
+            int updates = 0;
+            const int rc = hpb::dsolve_reliable(h, rows, nnz, row_ptr, col_ind, val32, val, b, x, r32, d32, *inner,
+                                                rel_tol, &ir, &updates);
+            if (rc != HPB_OK) return rc;
+            HPB_HIP(hipStreamSynchronize(h->stream));
+            res->seconds = now_s() - t0;
+            res->converged = ir.converged;
+            res->iters = ir.iters;
+            res->outer_iters = updates;
+            res->breakdown = ir.breakdown;
+            res->resid_norm = ir.resid_norm;
+            res->rhs_norm = bnorm;
+            res->true_rel_resid = true_rel_residual(h, A64, b, x, r);
+            return HPB_OK;
+        }
         for (;;) {
             hpb::check(hpb_csr_residualc_f64(h, rows, rows, nnz, row_ptr, col_ind, val, b, x, r), "ir: residual");
             double o[4];

@@ -25,7 +25,7 @@ class SolverParams(ctypes.Structure):
                 ("gram_schmidt", ctypes.c_int), ("early_exit", ctypes.c_int), ("reorthogonalization", ctypes.c_int),
                 ("resid_recalc_freq", ctypes.c_int), ("save_history", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("device_resident", ctypes.c_int), ("preconditioner", ctypes.c_int),
-                ("use_precond_resid", ctypes.c_int)]
+                ("use_precond_resid", ctypes.c_int), ("refinement", ctypes.c_int)]
 
 
 class SolverResult(ctypes.Structure):

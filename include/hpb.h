@@ -350,6 +350,11 @@ typedef struct hpb_solver_params {
                                 ILU(0) of A, csr/preconditioner.h; host-driven templates) */
     int use_precond_resid;   /* with a preconditioner: the monitor tests ||P r|| against ||P b||
                                 (set_use_precond_resid, gmres.h:396-405); 0: unpreconditioned residual */
+    int refinement;          /* hpb_csr_solve_ir_f64 with a device-resident BiCGStab(L) inner solver:
+                                1 (default) reliable updates -- one FP32 iteration that is never
+                                restarted; each time its recursive residual has dropped by the inner
+                                rel_tol, x64 += x32 and r = b - A x is recomputed in FP64 and fed back;
+                                0 restarted refinement (a fresh inner solve per sweep) */
 } hpb_solver_params;
 typedef struct hpb_solver_result {
     int converged;           /* monitor converged (true residual for GMRES restarts) */
@@ -371,7 +376,10 @@ int hpb_csr_solve_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr,
  * compensated FP32 dots inside, FP64 iterative refinement outside:
  *   r = b - A x (compensated FP64 rows); stop when ||r|| <= rel_tol ||b||;
  *   solve A d = r/||r|| in FP32 with `inner` (its rel_tol is the inner tolerance);
- *   x += ||r|| d in FP64. */
+ *   x += ||r|| d in FP64.
+ * With a BiCGStab(L) inner solver and inner->refinement = 1 the FP32 iteration is
+ * not restarted between updates (reliable updates); res->outer_iters then counts
+ * the FP64 updates and inner->max_iters caps the FP32 iterations. */
 int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
                          const double* val, const double* b, double* x, const hpb_solver_params* inner,
                          double rel_tol, int max_outer, hpb_solver_result* res);

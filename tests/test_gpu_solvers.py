@@ -156,3 +156,29 @@ def test_convdiff_128_gmres_and_bicgstab(handle):
         assert err < 1e-8
         dx.free()
     A.free()
+
+
+@pytest.mark.parametrize("delta", [0.3, 1e-2])
+@pytest.mark.parametrize("L", [1, 2, 4])
+def test_reliable_update_refinement(handle, L, delta):
+    """FP32 BiCGStab(L) never restarted, FP64 solution updated whenever the FP32
+    residual has dropped by delta: reaches 1e-12 in about as many iterations as the
+    FP64 solver, with several FP64 updates."""
+    Mh, A, b = _problem(handle, 24, 24, 24, S.CONVDIFF7)
+    say("reliable", L, delta)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    inner = S.params(solver=S.BICGSTABL, basis=L, rel_tol=delta, max_iters=5000, refinement=1)
+    res = S.solve_ir(handle, A, db, dx, inner, rel_tol=1e-12, max_outer=100)
+    assert res["converged"] == 1 and res["true_rel_resid"] <= 1e-12
+    assert _host_rel(Mh, b, dx.numpy()) <= 1.1e-12
+    assert res["outer_iters"] >= 2
+    dx.upload(np.zeros_like(b))
+    ref = S.solve(handle, A, db, dx, S.params(solver=S.BICGSTABL, basis=L, rel_tol=1e-12, max_iters=5000))
+    assert res["iters"] <= 2 * ref["iters"] + 4 * L, (res["iters"], ref["iters"])
+    # restarted refinement with the same inner solver still works
+    dx.upload(np.zeros_like(b))
+    inner0 = S.params(solver=S.BICGSTABL, basis=L, rel_tol=1e-4, max_iters=5000, refinement=0)
+    res0 = S.solve_ir(handle, A, db, dx, inner0, rel_tol=1e-12, max_outer=30)
+    assert res0["converged"] == 1 and res0["true_rel_resid"] <= 1e-12
+    A.free()
