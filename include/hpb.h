@@ -107,6 +107,18 @@ int hpb_nrm2c_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
 int hpb_nrm2c_f64_async(hpb_handle_t h, size_t n, const double* x, double* dev_out4);
 int hpb_nrm2c_f32_async(hpb_handle_t h, size_t n, const float* x, float* dev_out4);
 
+/* ---- complex dot products (interleaved re, im; n complex elements) -------- */
+/* x^H y = sum conj(x_i) y_i: the complex instantiations of gpu_reduction_ogita::dot
+ * (gpu_reduction_ogita_impl_functions.cuh:126-195: four TwoProds, a TwoSum per
+ * component, error terms (e1 + e2) + t).  out6 = { s_re, c_re, s_im, c_im,
+ * fl(s_re + c_re), fl(s_im + c_im) }.  The plain form accumulates with fma.
+ * The compensated 2-norm of a complex vector is hpb_nrm2c_* over its 2n reals. */
+int hpb_zdot2_f64(hpb_handle_t h, size_t n, const double* x, const double* y, double* out6);
+int hpb_zdot2_f32(hpb_handle_t h, size_t n, const float* x, const float* y, float* out6);
+int hpb_zdot_f64(hpb_handle_t h, size_t n, const double* x, const double* y, double* out6);
+int hpb_zdot_f32(hpb_handle_t h, size_t n, const float* x, const float* y, float* out6);
+int hpb_zdot2_f64_async(hpb_handle_t h, size_t n, const double* x, const double* y, double* dev_out6);
+
 /* ---- plain (uncompensated) reductions ----------------------------------- */
 /* gpu_reduction::{dot,sum,asum} (gpu_reduction.h:71-90) and the cuBLAS
  * standard-precision calls of gpu_vector_operations (cublas_wrap.h:525-591).

@@ -81,6 +81,10 @@ def lib():
          [ctypes.c_int, _sz, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp])
     _sig(L, "orc_hpb_order_reduce_f32", ctypes.c_int,
          [ctypes.c_int, _sz, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp])
+    _sig(L, "orc_zdot_order", ctypes.c_int,
+         [ctypes.c_int, _sz, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp])
+    _sig(L, "orc_zdot_order_f32", ctypes.c_int,
+         [ctypes.c_int, _sz, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp])
     vp = ctypes.c_void_p
     _sig(L, "orc_map", None, [ctypes.c_int, _sz, vp, vp, vp, vp, vp, vp])
     _sig(L, "orc_map_f32", None, [ctypes.c_int, _sz, vp, vp, vp, vp, vp, vp])
@@ -223,6 +227,21 @@ def hpb_order_reduce(op, x, y=None, cfg=(256, 4, 1, 2)):
     out = np.zeros(3, dtype=x.dtype)
     f = lib().orc_hpb_order_reduce if x.dtype == np.float64 else lib().orc_hpb_order_reduce_f32
     rc = f(REDUCE_OPS[op], x.size, ptr(x), ptr(y), int(cfg[0]), int(cfg[1]), int(cfg[2]), int(cfg[3]), ptr(out))
+    if rc:
+        raise ValueError("bad launch shape %r" % (cfg,))
+    return out
+
+
+def zdot_order(x, y, comp=True, cfg=(256, 4, 1)):
+    """x^H y of complex vectors in the GPU library's order for launch shape
+    cfg = (block, unroll, grid): (s_re, c_re, s_im, c_im, re, im)."""
+    x = np.ascontiguousarray(x)
+    y = np.ascontiguousarray(y)
+    real = np.float64 if x.dtype == np.complex128 else np.float32
+    xr, yr = x.view(real), y.view(real)
+    out = np.zeros(6, dtype=real)
+    f = lib().orc_zdot_order if real == np.float64 else lib().orc_zdot_order_f32
+    rc = f(1 if comp else 0, x.size, ptr(xr), ptr(yr), int(cfg[0]), int(cfg[1]), int(cfg[2]), ptr(out))
     if rc:
         raise ValueError("bad launch shape %r" % (cfg,))
     return out

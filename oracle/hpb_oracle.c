@@ -988,6 +988,76 @@ HP_DEFINE(double, , two_sum, two_prod, fma, fabs)
 HP_DEFINE(float, _f32, two_sumf, two_prodf, fmaf, fabsf)
 
 /* ------------------------------------------------------------------------ */
+/* Complex dot x^H y in the library's order (hpb_zdot.hip)                   */
+/* ------------------------------------------------------------------------ */
+/* Per element: the four real products of conj(a) b by TwoProd, a TwoSum per
+ * component, error term (e1 + e2) + t, as in the reference's complex TwoProd
+ * (gpu_reduction_ogita_impl_functions.cuh:126-195); accumulation as the real Dot2.
+ * Tree: the same tiles, butterflies and block-order merge as orc_hpb_order_reduce
+ * with vector width 1 complex element.  comp = 0: fma accumulation.
+ * out6 = { s_re, c_re, s_im, c_im, fl(s_re + c_re), fl(s_im + c_im) }. */
+#define ZD_DEFINE(T, SUF, TWO_SUM, TWO_PROD, FMA)                                                  \
+static inline void zd_acc##SUF(int comp, T *sr, T *cr, T *si, T *ci, T ar, T ai, T br, T bi)       \
+{                                                                                                 \
+    if (comp) {                                                                                   \
+        T e1, e2, e3, e4, t1, t2, t;                                                              \
+        T p1 = TWO_PROD(ar, br, &e1), p2 = TWO_PROD(ai, bi, &e2);                                 \
+        T p3 = TWO_PROD(ar, bi, &e3), p4 = TWO_PROD(-ai, br, &e4);                                \
+        T pr = TWO_SUM(p1, p2, &t1), pir = (e1 + e2) + t1;                                        \
+        T pi = TWO_SUM(p3, p4, &t2), pii = (e3 + e4) + t2;                                        \
+        *sr = TWO_SUM(*sr, pr, &t); *cr = *cr + (pir + t);                                        \
+        *si = TWO_SUM(*si, pi, &t); *ci = *ci + (pii + t);                                        \
+    } else {                                                                                      \
+        *sr = FMA(ar, br, *sr); *sr = FMA(ai, bi, *sr);                                           \
+        *si = FMA(ar, bi, *si); *si = FMA(-ai, br, *si);                                          \
+    }                                                                                             \
+}                                                                                                 \
+ORC_API int orc_zdot_order##SUF(int comp, size_t n, const T *x, const T *y, int block, int unroll, \
+                                int grid, T *out6)                                                \
+{                                                                                                 \
+    if (block % 64 || block <= 0 || unroll <= 0 || grid <= 0) return 1;                          \
+    size_t tile = (size_t)block * (size_t)unroll, nfull = n / tile;                               \
+    T *s = (T *)calloc((size_t)block, sizeof(T)), *c = (T *)calloc((size_t)block, sizeof(T));     \
+    T *s2 = (T *)calloc((size_t)block, sizeof(T)), *c2 = (T *)calloc((size_t)block, sizeof(T));   \
+    T *ps = (T *)calloc((size_t)grid * 4, sizeof(T));                                             \
+    for (int b = 0; b < grid; ++b) {                                                              \
+        for (int t = 0; t < block; ++t) {                                                         \
+            T sr = 0, cr = 0, si = 0, ci = 0;                                                     \
+            for (size_t tl = (size_t)b; tl < nfull; tl += (size_t)grid)                           \
+                for (int u = 0; u < unroll; ++u) {                                                \
+                    size_t e = tl * tile + (size_t)t + (size_t)u * (size_t)block;                 \
+                    zd_acc##SUF(comp, &sr, &cr, &si, &ci, x[2 * e], x[2 * e + 1], y[2 * e], y[2 * e + 1]); \
+                }                                                                                 \
+            if ((size_t)b == nfull % (size_t)grid)                                                \
+                for (int u = 0; u < unroll; ++u) {                                                \
+                    size_t e = nfull * tile + (size_t)t + (size_t)u * (size_t)block;              \
+                    if (e < n) zd_acc##SUF(comp, &sr, &cr, &si, &ci, x[2 * e], x[2 * e + 1], y[2 * e], y[2 * e + 1]); \
+                }                                                                                 \
+            s[t] = sr; c[t] = cr; s2[t] = si; c2[t] = ci;                                         \
+        }                                                                                         \
+        hp_block_reduce##SUF(comp, block, s, c, &ps[4 * b], &ps[4 * b + 1]);                      \
+        hp_block_reduce##SUF(comp, block, s2, c2, &ps[4 * b + 2], &ps[4 * b + 3]);                \
+    }                                                                                             \
+    for (int t = 0; t < block; ++t) {                                                             \
+        T a = 0, ac = 0, bq = 0, bc = 0;                                                          \
+        for (int i = t; i < grid; i += block) {                                                   \
+            hp_merge##SUF(comp, &a, &ac, ps[4 * i], ps[4 * i + 1]);                               \
+            hp_merge##SUF(comp, &bq, &bc, ps[4 * i + 2], ps[4 * i + 3]);                          \
+        }                                                                                         \
+        s[t] = a; c[t] = ac; s2[t] = bq; c2[t] = bc;                                              \
+    }                                                                                             \
+    hp_block_reduce##SUF(comp, block, s, c, &out6[0], &out6[1]);                                  \
+    hp_block_reduce##SUF(comp, block, s2, c2, &out6[2], &out6[3]);                                \
+    out6[4] = comp ? out6[0] + out6[1] : out6[0];                                                 \
+    out6[5] = comp ? out6[2] + out6[3] : out6[2];                                                 \
+    free(s); free(c); free(s2); free(c2); free(ps);                                               \
+    return 0;                                                                                     \
+}
+
+ZD_DEFINE(double, , two_sum, two_prod, fma)
+ZD_DEFINE(float, _f32, two_sumf, two_prodf, fmaf)
+
+/* ------------------------------------------------------------------------ */
 /* Elementwise operations with the rounding the library documents            */
 /* ------------------------------------------------------------------------ */
 /* gpu_vector_operations_impl.cuh:226-484 computes these as written and leaves
