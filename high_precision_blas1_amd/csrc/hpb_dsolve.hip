@@ -43,6 +43,7 @@ constexpr int MAXL = 8;        // BiCGStab(L) basis in the device path
 struct dstate {
     double tol, bnorm, rnorm;
     double rlast;   // reliable updates: FP64 true residual norm at the last update
+    int restart;    // reliable updates: the FP32 iteration restarts from the new residual
     double rho, alpha, omega;
     int iters, max_iters, min_iters, cycles;
     int done, stop, converged, breakdown, invalid;
@@ -235,20 +236,36 @@ __global__ __launch_bounds__(DB) void k_gm_dot(size_t n, int k, basis_pack<T> V,
                                                T* __restrict__ coef, int pass, const int* __restrict__ stop)
 {
     if (*stop) return;
+    using Vt = typename dev::vec16<T>::type;   // the vectors come from hipMalloc: 16-byte aligned
+    constexpr int W = dev::vec16<T>::W;
     const unsigned G = gridDim.x;
     const int j0 = blockIdx.y * KC;
     const int kc = (k - j0) < KC ? (k - j0) : KC;
     T s[KC], c[KC];
 #pragma unroll
     for (int q = 0; q < KC; ++q) s[q] = c[q] = T(0);
-    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)G * DB) {
-        const T a = w[i];
+    const size_t nv = n / W;
+    const Vt* wv = reinterpret_cast<const Vt*>(w);
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < nv; i += (size_t)G * DB) {
+        const Vt a = wv[i];
 #pragma unroll
         for (int q = 0; q < KC; ++q)
             if (q < kc) {
-                const T v = V.p[j0 + q][i];
-                if (COMP) dev::dot2_step<T>(s[q], c[q], v, a);
-                else s[q] = dev::fma_rn<T>(v, a, s[q]);
+                const Vt v = reinterpret_cast<const Vt*>(V.p[j0 + q])[i];
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    if (COMP) dev::dot2_step<T>(s[q], c[q], dev::vget<T>(v, e), dev::vget<T>(a, e));
+                    else s[q] = dev::fma_rn<T>(dev::vget<T>(v, e), dev::vget<T>(a, e), s[q]);
+                }
+            }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {   // the < W trailing elements
+        const size_t i = nv * W + threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < KC; ++q)
+            if (q < kc) {
+                if (COMP) dev::dot2_step<T>(s[q], c[q], V.p[j0 + q][i], w[i]);
+                else s[q] = dev::fma_rn<T>(V.p[j0 + q][i], w[i], s[q]);
             }
     }
 #pragma unroll
@@ -295,11 +312,44 @@ __global__ __launch_bounds__(DB) void k_gm_upd(size_t n, int k, basis_pack<T> V,
                                                dstate* __restrict__ st)
 {
     if (st->stop) return;
+    using Vt = typename dev::vec16<T>::type;
+    constexpr int W = dev::vec16<T>::W;
     __shared__ T sc[MAXB];
     for (int q = threadIdx.x; q < k; q += DB) sc[q] = coef[q];
     __syncthreads();
     T ns = T(0), nc = T(0);
-    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+    const size_t nv = n / W;
+    Vt* wv = reinterpret_cast<Vt*>(w);
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < nv; i += (size_t)gridDim.x * DB) {
+        Vt a = wv[i];
+        T s[W], c[W];
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            s[e] = dev::vget<T>(a, e);
+            c[e] = T(0);
+        }
+        for (int q = 0; q < k; ++q) {
+            const Vt v = reinterpret_cast<const Vt*>(V.p[q])[i];
+            const T cq = sc[q];
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                if (COMP) dev::dot2_step<T>(s[e], c[e], cq, dev::vget<T>(v, e));
+                else s[e] = dev::fma_rn<T>(cq, dev::vget<T>(v, e), s[e]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            const T r = COMP ? s[e] + c[e] : s[e];
+            reinterpret_cast<T*>(&a)[e] = r;
+            if (NORM) {
+                if (COMP) dev::dot2_step<T>(ns, nc, r, r);
+                else ns = dev::fma_rn<T>(r, r, ns);
+            }
+        }
+        wv[i] = a;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - nv * W) {   // the < W trailing elements
+        const size_t i = nv * W + threadIdx.x;
         T s = w[i], c = T(0);
         for (int q = 0; q < k; ++q) {
             if (COMP) dev::dot2_step<T>(s, c, sc[q], V.p[q][i]);
@@ -579,14 +629,18 @@ __global__ __launch_bounds__(DB) void k_rel_flush(size_t n, float* __restrict__ 
     }
 }
 
-// the FP64 true residual norm: monitor check, new reference for delta, resume
+// the FP64 true residual norm: monitor check, new reference for delta, resume.
+// When the update did not lower the true residual the FP32 recurrences have lost
+// their accuracy: the iteration restarts from the new residual (r~, u_0, rho,
+// alpha, omega as at the start).
 __global__ void k_rel_after(dstate* st, const double* n4, double* hist, int first)
 {
     if (st->done) return;
     const double rn = sqrt(dval(n4));
+    st->restart = first || !(rn < st->rlast);
     st->rlast = rn;
     monitor_check(st, hist, rn);
-    if (first) {
+    if (st->restart) {
         st->rho = 1.0;
         st->alpha = 0.0;
         st->omega = 1.0;
@@ -594,17 +648,18 @@ __global__ void k_rel_after(dstate* st, const double* n4, double* hist, int firs
     st->stop = st->done;
 }
 
-// r32 = (float) r64; on the first update also r~ = r32 / ||r32|| and u_0 = 0
+// r32 = (float) r64; on a restart also r~ = r32 / ||r32|| and u_0 = 0
 __global__ __launch_bounds__(DB) void k_rel_convert(size_t n, const double* __restrict__ r64, float* __restrict__ r32,
                                                     float* __restrict__ rt, float* __restrict__ u0,
-                                                    const dstate* __restrict__ st, int first)
+                                                    const dstate* __restrict__ st)
 {
     if (st->done) return;
     const double inv = st->rlast > 0.0 ? 1.0 / st->rlast : 1.0;
+    const bool restart = st->restart != 0;
     for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
         const float v = (float)r64[i];
         r32[i] = v;
-        if (first) {
+        if (restart) {
             rt[i] = (float)(r64[i] * inv);
             u0[i] = 0.0f;
         }
@@ -1027,7 +1082,7 @@ private:
         DS_CHECK(nrm2_async<double>(h, n, r64.p, n4d.p, true, &S->done));
         hipLaunchKernelGGL(k_rel_after, dim3(1), dim3(1), 0, st, S, n4d.p, hist(), first);
         hipLaunchKernelGGL(k_rel_convert, dim3(vg), dim3(DB), 0, st, n, r64.p, (float*)V[0], (float*)rt.p,
-                           (float*)Ub[0].p, S, first);
+                           (float*)Ub[0].p, S);
         HPB_LAUNCH_CHECK();
         return HPB_OK;
     }
