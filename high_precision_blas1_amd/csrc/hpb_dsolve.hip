@@ -596,6 +596,272 @@ __global__ void k_bs_check(dstate* st, const T* n4, double* hist)
     monitor_check(st, hist, sqrt(dval(n4)));
 }
 
+// ---- fused BiCGStab(L) cycle: reductions finish with their scalar step --------
+
+// Finish NQ reductions: per-block totals to part, the block taking the last ticket
+// merges them in block order; true in thread 0 of that block, with out[q] = s + c.
+template <typename T, int NQ, bool COMP>
+__device__ bool reduce_finish(T (&s)[NQ], T (&c)[NQ], T* part, unsigned* ticket, double (&out)[NQ])
+{
+    const unsigned G = gridDim.x;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        dev::block_reduce<T, COMP, DB>(s[q], c[q]);
+        if (threadIdx.x == 0) {
+            st_ag<T>(&part[2 * ((size_t)q * G + blockIdx.x)], s[q]);
+            st_ag<T>(&part[2 * ((size_t)q * G + blockIdx.x) + 1], c[q]);
+        }
+    }
+    if (!last_block(ticket, G)) return false;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        T fs = T(0), fc = T(0);
+        for (unsigned b = threadIdx.x; b < G; b += DB) {
+            const T ps = ld_ag<T>(&part[2 * ((size_t)q * G + b)]);
+            const T pc = ld_ag<T>(&part[2 * ((size_t)q * G + b) + 1]);
+            if (COMP) dev::merge<T>(fs, fc, ps, pc);
+            else fs = fs + ps;
+        }
+        dev::block_reduce<T, COMP, DB>(fs, fc);
+        if (threadIdx.x == 0) out[q] = (double)fs + (double)fc;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return threadIdx.x == 0;
+}
+
+// scalar steps of bicgstabl.h:199-210 on merged dot products
+template <typename T>
+__device__ void epi_alpha(dstate* st, double sg, T* coef)
+{
+    if (sg == 0.0 || !finite(sg)) {
+        breakdown_stop(st);
+        return;
+    }
+    st->alpha = st->rho / sg;
+    coef[1] = (T)st->alpha;
+}
+template <typename T>
+__device__ void epi_beta(dstate* st, double rho1, int first, T* coef)
+{
+    double rho = st->rho;
+    if (first) rho = -st->omega * rho;
+    if (rho == 0.0 || !finite(rho)) {
+        breakdown_stop(st);
+        return;
+    }
+    coef[0] = (T)(st->alpha * rho1 / rho);
+    st->rho = rho1;
+}
+
+enum { EPI_NONE = 0, EPI_ALPHA, EPI_BETA, EPI_CHECK, EPI_REL };
+
+// end of a cycle: d0 = (r_0, r~), d1 = (r_0, r_0)
+template <typename T>
+__device__ void epi_cycle(int kind, dstate* st, double d0, double d1, T* coef, double* hist, double delta)
+{
+    st->iters += st->basis;
+    st->cycles++;
+    const double rn = sqrt(d1);
+    if (kind == EPI_CHECK) {
+        monitor_check(st, hist, rn);
+        if (st->done) {
+            st->stop = 1;
+            return;
+        }
+    } else {   // reliable updates: pause for an FP64 update, or continue
+        st->rnorm = rn;
+        if (!finite(rn)) {
+            st->invalid = 1;
+            st->done = 1;
+            st->stop = 1;
+            return;
+        }
+        if (rn <= delta * st->rlast || rn <= st->tol || st->iters >= st->max_iters) {
+            st->stop = 1;
+            return;
+        }
+    }
+    epi_beta<T>(st, d0, 1, coef);
+}
+
+// y = A x (MODE 0) or y = b - A x (MODE 1), LPR lanes per row (Dot2 rows when
+// CROWS), with the dot products of y that the cycle needs next:
+//   EPI_ALPHA  (y, r~) -> alpha;  EPI_BETA (y, r~) -> beta of the next BiCG step;
+//   EPI_CHECK / EPI_REL (y, r~), (y, y) -> monitor check and beta of the next cycle.
+template <typename T, int LPR, bool CROWS, bool COMP, int MODE, int EPI>
+__global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict__ rp, const int* __restrict__ col,
+                                                const T* __restrict__ val, const T* __restrict__ x,
+                                                const T* __restrict__ b, T* __restrict__ y, const T* __restrict__ rt,
+                                                T* __restrict__ part, unsigned* __restrict__ ticket,
+                                                dstate* __restrict__ st, T* __restrict__ coef, double* hist,
+                                                double delta, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    constexpr int NQ = EPI == EPI_NONE ? 0 : (EPI == EPI_ALPHA || EPI == EPI_BETA ? 1 : 2);
+    T ds[NQ > 0 ? NQ : 1], dc[NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) ds[q] = dc[q] = T(0);
+    const int lane = threadIdx.x % LPR;
+    const long row_stride = (long)gridDim.x * (DB / LPR);
+    for (long r = ((long)blockIdx.x * DB + threadIdx.x) / LPR; r < rows; r += row_stride) {
+        const int beg = rp[r], end = rp[r + 1];
+        T s = T(0), c = T(0);
+        if (MODE == 1 && lane == 0) s = b[r];
+        for (int k = beg + lane; k < end; k += LPR) {
+            const T a = MODE == 1 ? -val[k] : val[k];
+            if (CROWS) dev::dot2_step<T>(s, c, a, x[col[k]]);
+            else s = dev::fma_rn<T>(a, x[col[k]], s);
+        }
+#pragma unroll
+        for (int m = LPR / 2; m >= 1; m >>= 1) {
+            const T s2 = __shfl_xor(s, m, 64);
+            if (CROWS) {
+                const T c2 = __shfl_xor(c, m, 64);
+                dev::merge<T>(s, c, s2, c2);
+            } else {
+                s = s + s2;
+            }
+        }
+        if (lane == 0) {
+            const T v = CROWS ? s + c : s;
+            y[r] = v;
+            if constexpr (NQ >= 1) {
+                if (COMP) dev::dot2_step<T>(ds[0], dc[0], v, rt[r]);
+                else ds[0] = dev::fma_rn<T>(v, rt[r], ds[0]);
+            }
+            if constexpr (NQ >= 2) {
+                if (COMP) dev::dot2_step<T>(ds[1], dc[1], v, v);
+                else ds[1] = dev::fma_rn<T>(v, v, ds[1]);
+            }
+        }
+    }
+    if constexpr (NQ > 0) {
+        double out[NQ];
+        if (!reduce_finish<T, NQ, COMP>(ds, dc, part, ticket, out)) return;
+        if (EPI == EPI_ALPHA) epi_alpha<T>(st, out[0], coef);
+        else if (EPI == EPI_BETA) epi_beta<T>(st, out[0], 0, coef);
+        else epi_cycle<T>(EPI, st, out[0], out[NQ - 1], coef, hist, delta);
+    }
+}
+
+// MR part in one pass: (r_0, r_k) and (r_i, r_k), 1 <= i <= k <= L, then the
+// normal equations by Cholesky in FP64 (as k_bs_mr); L <= 4.
+template <typename T, int L, bool COMP>
+__global__ __launch_bounds__(DB) void k_ds_gram(size_t n, bs_pack<T> P, T* __restrict__ part,
+                                                unsigned* __restrict__ ticket, dstate* __restrict__ st,
+                                                T* __restrict__ coef, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    constexpr int NP = L + L * (L + 1) / 2;
+    T s[NP], c[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) s[q] = c[q] = T(0);
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        T r[L + 1];
+#pragma unroll
+        for (int k = 0; k <= L; ++k) r[k] = P.r[k][i];
+        int q = 0;
+#pragma unroll
+        for (int k = 1; k <= L; ++k) {
+#pragma unroll
+            for (int i2 = 0; i2 <= k; ++i2) {
+                if (COMP) dev::dot2_step<T>(s[q], c[q], r[i2], r[k]);
+                else s[q] = dev::fma_rn<T>(r[i2], r[k], s[q]);
+                ++q;
+            }
+        }
+    }
+    double out[NP];
+    if (!reduce_finish<T, NP, COMP>(s, c, part, ticket, out)) return;
+    double G[L][L], cv[L], yv[L];
+    int q = 0;
+    for (int k = 1; k <= L; ++k)
+        for (int i2 = 0; i2 <= k; ++i2) {
+            const double v = out[q++];
+            if (i2 == 0) cv[k - 1] = v;
+            else {
+                G[i2 - 1][k - 1] = v;
+                G[k - 1][i2 - 1] = v;
+            }
+        }
+    for (int j = 0; j < L; ++j) {
+        double d = G[j][j];
+        for (int k = 0; k < j; ++k) d -= G[k][j] * G[k][j];
+        if (!(d > 0.0) || !finite(d)) {
+            breakdown_stop(st);
+            return;
+        }
+        d = sqrt(d);
+        G[j][j] = d;
+        for (int i2 = j + 1; i2 < L; ++i2) {
+            double v = G[j][i2];
+            for (int k = 0; k < j; ++k) v -= G[k][j] * G[k][i2];
+            G[j][i2] = v / d;
+        }
+    }
+    for (int i2 = 0; i2 < L; ++i2) {
+        double v = cv[i2];
+        for (int k = 0; k < i2; ++k) v -= G[k][i2] * yv[k];
+        yv[i2] = v / G[i2][i2];
+    }
+    for (int i2 = L - 1; i2 >= 0; --i2) {
+        double v = yv[i2];
+        for (int k = i2 + 1; k < L; ++k) v -= G[i2][k] * yv[k];
+        yv[i2] = v / G[i2][i2];
+    }
+    st->omega = yv[L - 1];
+    for (int j = 0; j < L; ++j) coef[2 + j] = (T)yv[j];
+}
+
+// MR update of x, r_0, u_0 (as k_bs_upd_mr) with (r_0, r~), (r_0, r_0) of the new
+// r_0 and the end-of-cycle step (monitor check or reliable-update trigger, beta)
+template <typename T, bool COMP, int EPI>
+__global__ __launch_bounds__(DB) void k_ds_upd_mr(size_t n, int L, bs_pack<T> P, T* __restrict__ x,
+                                                  const T* __restrict__ rt, T* __restrict__ coef,
+                                                  T* __restrict__ part, unsigned* __restrict__ ticket,
+                                                  dstate* __restrict__ st, double* hist, double delta,
+                                                  const int* __restrict__ skip)
+{
+    if (*skip) return;
+    __shared__ T gm[MAXL];
+    if (threadIdx.x < (unsigned)L) gm[threadIdx.x] = coef[2 + threadIdx.x];
+    __syncthreads();
+    T ds[2] = {T(0), T(0)}, dc[2] = {T(0), T(0)};
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
+        T sx = x[i], cx = T(0), sr = P.r[0][i], cr = T(0), su = P.u[0][i], cu = T(0);
+        T rprev = sr;
+        for (int j = 1; j <= L; ++j) {
+            const T g = gm[j - 1];
+            const T rj = P.r[j][i];
+            const T uj = P.u[j][i];
+            if (COMP) {
+                dev::dot2_step<T>(sx, cx, g, rprev);
+                dev::dot2_step<T>(sr, cr, -g, rj);
+                dev::dot2_step<T>(su, cu, -g, uj);
+            } else {
+                sx = dev::fma_rn<T>(g, rprev, sx);
+                sr = dev::fma_rn<T>(-g, rj, sr);
+                su = dev::fma_rn<T>(-g, uj, su);
+            }
+            rprev = rj;
+        }
+        const T r0 = COMP ? sr + cr : sr;
+        x[i] = COMP ? sx + cx : sx;
+        P.r[0][i] = r0;
+        P.u[0][i] = COMP ? su + cu : su;
+        if (COMP) {
+            dev::dot2_step<T>(ds[0], dc[0], r0, rt[i]);
+            dev::dot2_step<T>(ds[1], dc[1], r0, r0);
+        } else {
+            ds[0] = dev::fma_rn<T>(r0, rt[i], ds[0]);
+            ds[1] = dev::fma_rn<T>(r0, r0, ds[1]);
+        }
+    }
+    double out[2];
+    if (!reduce_finish<T, 2, COMP>(ds, dc, part, ticket, out)) return;
+    epi_cycle<T>(EPI, st, out[0], out[1], coef, hist, delta);
+}
+
 // ---- reliable updates (FP32 iterations, FP64 solution and true residual)
 
 // after a cycle: the FP32 recursive residual decides whether the FP64 solution is
@@ -970,8 +1236,59 @@ private:
         return HPB_OK;
     }
 
+    // fused launchers of the BiCGStab(L) cycle (L <= 4, residual recomputed every
+    // cycle or never, or reliable updates)
+    template <int MODE, int EPI>
+    void ds_spmv(const T* x, const T* b, T* y, const T* rtv, const int* skip)
+    {
+        hpb_handle_t h = P.h;
+        const double mean = P.rows > 0 ? (double)P.nnz / P.rows : 1.0;
+        const int lpr = mean <= 6.0 ? 4 : (mean <= 12.0 ? 8 : 16);
+        const long per_block = DB / lpr;
+        long grid = (P.rows + per_block - 1) / per_block;
+        const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;
+        grid = std::max(1L, std::min(grid, cap));
+        const bool cr = prm.compensated_spmv != 0, comp = prm.high_precision != 0;
+        T* part = reinterpret_cast<T*>(h->part);
+        unsigned* tk = h->ticket + 3;
+#define DS_SPMV(L_, CR_, CO_)                                                                                \
+    hipLaunchKernelGGL((k_ds_spmv<T, L_, CR_, CO_, MODE, EPI>), dim3((unsigned)grid), dim3(DB), 0, h->stream,   \
+                       P.rows, P.rp, P.ci, P.val, x, b, y, rtv, part, tk, dst.p, coef.p, hist(), R.delta, skip)
+#define DS_SPMV_L(L_)                                                                                        \
+    do {                                                                                                     \
+        if (cr) { if (comp) DS_SPMV(L_, true, true); else DS_SPMV(L_, true, false); }                        \
+        else { if (comp) DS_SPMV(L_, false, true); else DS_SPMV(L_, false, false); }                         \
+    } while (0)
+        if (lpr == 4) DS_SPMV_L(4);
+        else if (lpr == 8) DS_SPMV_L(8);
+        else DS_SPMV_L(16);
+#undef DS_SPMV_L
+#undef DS_SPMV
+    }
+
+    bool fused_ok() const
+    {
+        return prm.basis <= 4 && (reliable || prm.resid_recalc_freq == 0 || prm.resid_recalc_freq == 1);
+    }
+
+    template <int L>
+    void ds_gram(bs_pack<T> bp, const int* skip)
+    {
+        hpb_handle_t h = P.h;
+        const size_t n = (size_t)P.rows;
+        T* part = reinterpret_cast<T*>(h->part);
+        if (prm.high_precision)
+            hipLaunchKernelGGL((k_ds_gram<T, L, true>), dim3(vgrid(h, n)), dim3(DB), 0, h->stream, n, bp, part,
+                               h->ticket + 3, dst.p, coef.p, skip);
+        else
+            hipLaunchKernelGGL((k_ds_gram<T, L, false>), dim3(vgrid(h, n)), dim3(DB), 0, h->stream, n, bp, part,
+                               h->ticket + 3, dst.p, coef.p, skip);
+    }
+
     int setup_bicgstabl()
     {
+        fused = fused_ok();
+        if (fused) return setup_bicgstabl_fused();
         hpb_handle_t h = P.h;
         hipStream_t st = h->stream;
         const size_t n = (size_t)P.rows;
@@ -1037,6 +1354,83 @@ private:
         return end_capture<T>(st, gg, rc);
     }
 
+    int setup_bicgstabl_fused()
+    {
+        hpb_handle_t h = P.h;
+        hipStream_t st = h->stream;
+        const size_t n = (size_t)P.rows;
+        const int L = prm.basis;
+        const bool comp = prm.high_precision != 0;
+        Vb.resize(L + 1);
+        Ub.resize(L + 1);
+        V.assign(L + 1, nullptr);
+        bs_pack<T> bp{};
+        for (int j = 0; j <= L; ++j) {
+            DS_CHECK(Vb[j].alloc(n));
+            DS_CHECK(Ub[j].alloc(n));
+            bp.r[j] = Vb[j].p;
+            bp.u[j] = Ub[j].p;
+            V[j] = Vb[j].p;
+        }
+        gstride = 4 * (L + 1);
+        DS_CHECK(rt.alloc(n));
+        DS_CHECK(n4.alloc(4));
+        DS_CHECK(da.alloc(4));
+        DS_CHECK(db.alloc(4));
+        DS_CHECK(gram.alloc((size_t)gstride * (L + 1)));
+        DS_CHECK(coef.alloc(2 + MAXL));
+        DS_CHECK(scal.alloc(1));
+        dstate* S = dst.p;
+        const int* skip = reliable ? &S->stop : &S->done;
+        const unsigned vg = vgrid(h, n);
+        T* part = reinterpret_cast<T*>(h->part);
+        const bool recalc = !reliable && prm.resid_recalc_freq == 1;
+        HPB_HIP(hipStreamSynchronize(st));
+        HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = [&]() -> int {
+            for (int j = 0; j < L; ++j) {
+                hipLaunchKernelGGL((k_bs_upd_u<T>), dim3(vg), dim3(DB), 0, st, n, j, bp, coef.p, skip);
+                ds_spmv<0, EPI_ALPHA>(Ub[j].p, nullptr, Ub[j + 1].p, rt.p, skip);
+                hipLaunchKernelGGL((k_bs_upd_r<T>), dim3(vg), dim3(DB), 0, st, n, j, bp, P.x, coef.p, skip);
+                if (j + 1 < L) ds_spmv<0, EPI_BETA>(V[j], nullptr, V[j + 1], rt.p, skip);
+                else ds_spmv<0, EPI_NONE>(V[j], nullptr, V[j + 1], rt.p, skip);
+            }
+            switch (L) {
+            case 1: ds_gram<1>(bp, skip); break;
+            case 2: ds_gram<2>(bp, skip); break;
+            case 3: ds_gram<3>(bp, skip); break;
+            default: ds_gram<4>(bp, skip); break;
+            }
+            const int epi = reliable ? EPI_REL : EPI_CHECK;
+            if (recalc) {
+                if (comp)
+                    hipLaunchKernelGGL((k_bs_upd_mr<T, true>), dim3(vg), dim3(DB), 0, st, n, L, bp, P.x, coef.p, skip);
+                else
+                    hipLaunchKernelGGL((k_bs_upd_mr<T, false>), dim3(vg), dim3(DB), 0, st, n, L, bp, P.x, coef.p, skip);
+                ds_spmv<1, EPI_CHECK>(P.x, P.b, V[0], rt.p, skip);
+            } else {
+                auto upd = [&](auto compc, auto epic) {
+                    hipLaunchKernelGGL((k_ds_upd_mr<T, decltype(compc)::value, decltype(epic)::value>), dim3(vg),
+                                       dim3(DB), 0, st, n, L, bp, P.x, rt.p, coef.p, part, h->ticket + 3, S, hist(),
+                                       R.delta, skip);
+                };
+                using CT = std::true_type;
+                using CF = std::false_type;
+                using EC = std::integral_constant<int, EPI_CHECK>;
+                using ER = std::integral_constant<int, EPI_REL>;
+                if (comp) {
+                    if (epi == EPI_REL) upd(CT{}, ER{});
+                    else upd(CT{}, EC{});
+                } else {
+                    if (epi == EPI_REL) upd(CF{}, ER{});
+                    else upd(CF{}, EC{});
+                }
+            }
+            return HPB_OK;
+        }();
+        return end_capture<T>(st, gg, rc);
+    }
+
     int run_bicgstabl(dstate* out)
     {
         hpb_handle_t h = P.h;
@@ -1055,6 +1449,7 @@ private:
         hipLaunchKernelGGL((k_scale_copy<T>), dim3(vg), dim3(DB), 0, st, n, V[0], rt.p, scal.p, &S->invalid);
         hipLaunchKernelGGL((k_fill<T>), dim3(vg), dim3(DB), 0, st, n, Ub[0].p, T(0));
         HPB_LAUNCH_CHECK();
+        DS_CHECK(fused_beta(&S->done));
         DS_CHECK(read_state<T>(h, S, out));
         if (out->done) return HPB_OK;
         const int max_cycles = prm.max_iters / L + 2;
@@ -1084,7 +1479,7 @@ private:
         hipLaunchKernelGGL(k_rel_convert, dim3(vg), dim3(DB), 0, st, n, r64.p, (float*)V[0], (float*)rt.p,
                            (float*)Ub[0].p, S);
         HPB_LAUNCH_CHECK();
-        return HPB_OK;
+        return fused_beta(&S->done);
     }
 
     int run_reliable(dstate* out)
@@ -1121,6 +1516,18 @@ private:
 
 public:
     int updates = 0;
+    bool fused = false;   // BiCGStab(L): the cycle graph finishes with beta for the next cycle
+
+    // fused cycle: beta and rho for the first BiCG step from the current r_0
+    int fused_beta(const int* skip)
+    {
+        if (!fused) return HPB_OK;
+        hpb_handle_t h = P.h;
+        DS_CHECK(dot2_async<T>(h, (size_t)P.rows, V[0], rt.p, da.p, prm.high_precision != 0, skip));
+        hipLaunchKernelGGL((k_bs_beta<T>), dim3(1), dim3(1), 0, h->stream, dst.p, da.p, coef.p, 1, skip);
+        HPB_LAUNCH_CHECK();
+        return HPB_OK;
+    }
 };
 
 }  // namespace
