@@ -10,9 +10,10 @@
 // Algorithm as in the reference: Arnoldi with Gram-Schmidt, Givens rotations on
 // the host Hessenberg (row-major H[j*m + k]), back substitution, x += V s, true
 // residual and monitor check at every restart.  Differences, all documented:
-//  * orthogonalization: MGS (the reference's, gmres.h:426-452) or CGS2 -- two
+//  * orthogonalization: MGS (the reference's, gmres.h:426-452), CGS2 -- two
 //    classical passes, each one fused multi-dot (reads w once) plus one fused
-//    multi-update, 2 host synchronisations per step instead of i + 2;
+//    multi-update, 2 host synchronisations per step instead of i + 2 -- or CGS
+//    with the second pass only when the first shrank w by more than 1/sqrt(2);
 //  * early_exit: leave the cycle when the Givens estimate |s_{i+1}| is below the
 //    tolerance (the reference runs all m steps; its early exit is commented out,
 //    gmres.h:459-474); convergence is still decided on the true residual;
@@ -32,7 +33,10 @@
 namespace numerical_algos {
 namespace lin_solvers {
 
-enum class gram_schmidt { mgs, cgs2 };
+// mgs: the reference's; cgs2: two classical passes; cgs_dgks: the second pass
+// only when the first one shrank w by more than 1/sqrt(2) (Daniel, Gragg,
+// Kaufman and Stewart 1976, "twice is enough").
+enum class gram_schmidt { mgs, cgs2, cgs_dgks };
 
 template <class LinearOperator, class Preconditioner, class VectorOperations, class Monitor, class Log,
           unsigned int GMRESMaxBasisSize = 100>
@@ -201,9 +205,18 @@ private:
                 H_[k * m + i] = alpha;
             }
         } else {
+            const bool selective = gs_ == gram_schmidt::cgs_dgks;
+            const T wn = selective ? vec_ops_->norm(w) : T(0);
             detail::multi_dot(*vec_ops_, i + 1, V_.data(), w, h1_.data());
             for (int k = 0; k <= i; ++k) h2_[k] = -h1_[k];
             detail::multi_axpy(*vec_ops_, i + 1, h2_.data(), V_.data(), w);
+            if (selective) {
+                const T wn1 = vec_ops_->norm(w);
+                if (wn1 * wn1 >= T(0.5) * wn * wn) {
+                    for (int k = 0; k <= i; ++k) H_[k * m + i] = h1_[k];
+                    return;
+                }
+            }
             detail::multi_dot(*vec_ops_, i + 1, V_.data(), w, h2_.data());
             for (int k = 0; k <= i; ++k) {
                 H_[k * m + i] = h1_[k] + h2_[k];

@@ -49,6 +49,8 @@ struct dstate {
     int done, stop, converged, breakdown, invalid;
     int cycle_len, hist_len, hist_cap, recalc_freq, skip_recalc;
     int early_exit, basis;
+    double wn2;   // GMRES selective reorthogonalisation: ||w||^2 before the projection
+    int skip1;    // ... 1 when the second Gram-Schmidt pass is not needed (or the cycle stopped)
 };
 
 template <typename T>
@@ -197,43 +199,71 @@ struct basis_pack {
     const T* p[MAXB];
 };
 
-template <typename T> struct ubits { using type = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned>; };
-template <typename T>
-__device__ __forceinline__ void st_ag(T* p, T v)
+// Reductions over many blocks end in per-block totals stored plainly in `part`;
+// a one-block kernel launched next merges them in a fixed order (block by block
+// per thread, then threads in order) and runs the scalar step.  A last-block
+// ticket would need an agent-scope release in every block -- an L2 write-back on
+// this multi-XCD part, about 20 ns serialised per block -- which costs more than
+// the extra launch once a grid has more than a few hundred blocks.
+template <typename T, bool COMP>
+__device__ __forceinline__ void part_out(T s, T c, T* __restrict__ part, size_t slot)
 {
-    using U = typename ubits<T>::type;
-    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T ld_ag(const T* p)
-{
-    using U = typename ubits<T>::type;
-    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT));
+    dev::block_reduce<T, COMP, DB>(s, c);
+    if (threadIdx.x == 0) {
+        part[2 * slot] = s;
+        part[2 * slot + 1] = c;
+    }
 }
 
-__device__ __forceinline__ bool last_block(unsigned* ticket, unsigned total)
+// One block of DB threads: columns 0..NQ-1 of G partials each merged (thread t
+// takes partials t, t + DB, ... of every column -- independent loads in flight --,
+// then the waves' butterflies and the wave totals in order); out valid in thread 0.
+template <typename T, bool COMP, int NQ>
+__device__ __forceinline__ void merge_columns(const T* __restrict__ part, unsigned G, double (&out)[NQ])
 {
-    __shared__ int am_last;
-    if (threadIdx.x == 0) {
-        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == total - 1);
+    T fs[NQ], fc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) fs[q] = fc[q] = T(0);
+    for (unsigned b = threadIdx.x; b < G; b += DB) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const T ps = part[2 * ((size_t)q * G + b)];
+            const T pc = part[2 * ((size_t)q * G + b) + 1];
+            if (COMP) dev::merge<T>(fs[q], fc[q], ps, pc);
+            else fs[q] = fs[q] + ps;
+        }
+    }
+    constexpr int NW = DB / 64;
+    __shared__ T ls[NW][NQ], lc[NW][NQ];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        dev::wave_reduce<T, COMP>(fs[q], fc[q]);
+        if (lane == 0) {
+            ls[wid][q] = fs[q];
+            lc[wid][q] = fc[q];
+        }
     }
     __syncthreads();
-    const bool last = am_last != 0;
-    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            T s = ls[0][q], c = lc[0][q];
+            for (int w = 1; w < NW; ++w) {
+                if (COMP) dev::merge<T>(s, c, ls[w][q], lc[w][q]);
+                else s = s + ls[w][q];
+            }
+            out[q] = (double)s + (double)c;
+        }
+    }
 }
 
-// One Gram-Schmidt projection pass for column j: h_i = (sig_i V_i, w), i <= j,
-// compensated or plain, KC columns per blockIdx.y chunk.  The block that takes
-// the last ticket merges each column's partials in block order and writes
-// H(i, j) (pass 0) or adds to it (pass 1), and coef_i = -h_i sig_i for the update.
+// One Gram-Schmidt projection pass for column j: per-block partials of
+// (V_i, w), i <= j, compensated or plain, KC columns per blockIdx.y chunk;
+// k_gm_hfin merges them.
 template <typename T, int KC, bool COMP>
 __global__ __launch_bounds__(DB) void k_gm_dot(size_t n, int k, basis_pack<T> V, const T* __restrict__ w,
-                                               const T* __restrict__ sig, T* __restrict__ part,
-                                               unsigned* __restrict__ ticket, double* __restrict__ Hj,
-                                               T* __restrict__ coef, int pass, const int* __restrict__ stop)
+                                               T* __restrict__ part, const int* __restrict__ stop)
 {
     if (*stop) return;
     using Vt = typename dev::vec16<T>::type;   // the vectors come from hipMalloc: 16-byte aligned
@@ -270,26 +300,35 @@ __global__ __launch_bounds__(DB) void k_gm_dot(size_t n, int k, basis_pack<T> V,
     }
 #pragma unroll
     for (int q = 0; q < KC; ++q)
-        if (q < kc) {
-            T ss = s[q], cc = c[q];
-            dev::block_reduce<T, COMP, DB>(ss, cc);
-            if (threadIdx.x == 0) {
-                st_ag<T>(&part[2 * ((size_t)(j0 + q) * G + blockIdx.x)], ss);
-                st_ag<T>(&part[2 * ((size_t)(j0 + q) * G + blockIdx.x) + 1], cc);
-            }
-        }
-    if (!last_block(ticket, G * gridDim.y)) return;
+        if (q < kc) part_out<T, COMP>(s[q], c[q], part, (size_t)(j0 + q) * G + blockIdx.x);
+}
+
+// h_i = sig_i (V_i, w) from G partials per column: H(i, j) (pass 0) or += (pass 1),
+// and coef_i = -h_i sig_i for the update.  One block, a wave per column.
+// With kk = k + 1 the last column is (w, w) (pass 0 of the selective
+// reorthogonalisation), kept unscaled in st->wn2.
+template <typename T, bool COMP>
+__global__ __launch_bounds__(1024) void k_gm_hfin(const T* __restrict__ part, unsigned G, int k, int kk,
+                                                  const T* __restrict__ sig, double* __restrict__ Hj,
+                                                  T* __restrict__ coef, int pass, const int* __restrict__ stop,
+                                                  dstate* __restrict__ st)
+{
+    if (*stop) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int col = wid; col < k; col += DB / 64) {
+    for (int col = wid; col < kk; col += 1024 / 64) {
         T fs = T(0), fc = T(0);
         for (unsigned b = lane; b < G; b += 64) {
-            const T ps = ld_ag<T>(&part[2 * ((size_t)col * G + b)]);
-            const T pc = ld_ag<T>(&part[2 * ((size_t)col * G + b) + 1]);
+            const T ps = part[2 * ((size_t)col * G + b)];
+            const T pc = part[2 * ((size_t)col * G + b) + 1];
             if (COMP) dev::merge<T>(fs, fc, ps, pc);
             else fs = fs + ps;
         }
         dev::wave_reduce<T, COMP>(fs, fc);
         if (lane == 0) {
+            if (col == k) {
+                st->wn2 = (double)fs + (double)fc;
+                continue;
+            }
             const double sg = (double)sig[col];
             const double h = ((double)fs + (double)fc) * sg;
             if (pass == 0) Hj[col] = h;
@@ -297,21 +336,16 @@ __global__ __launch_bounds__(DB) void k_gm_dot(size_t n, int k, basis_pack<T> V,
             coef[col] = (T)(-h * sg);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // w += sum_{i<k} coef_i V_i (Dot2 over the terms of each element when COMP);
-// NORM: also ||w||^2 (compensated), and the last block runs the Givens step for
-// column j with h_{j+1,j} = ||w||.
+// NORM: also per-block partials of ||w||^2 (compensated) for k_gm_nfin.
 template <typename T, bool COMP, bool NORM>
 __global__ __launch_bounds__(DB) void k_gm_upd(size_t n, int k, basis_pack<T> V, const T* __restrict__ coef,
-                                               T* __restrict__ w, T* __restrict__ part, unsigned* __restrict__ ticket,
-                                               int j, int m, double* __restrict__ H, int ldh, double* __restrict__ cs,
-                                               double* __restrict__ sn, double* __restrict__ g, T* __restrict__ sig,
-                                               dstate* __restrict__ st)
+                                               T* __restrict__ w, T* __restrict__ part,
+                                               const int* __restrict__ skip)
 {
-    if (st->stop) return;
+    if (*skip) return;
     using Vt = typename dev::vec16<T>::type;
     constexpr int W = dev::vec16<T>::W;
     __shared__ T sc[MAXB];
@@ -362,25 +396,44 @@ __global__ __launch_bounds__(DB) void k_gm_upd(size_t n, int k, basis_pack<T> V,
             else ns = dev::fma_rn<T>(r, r, ns);
         }
     }
-    if (!NORM) return;
-    dev::block_reduce<T, COMP, DB>(ns, nc);
-    const unsigned G = gridDim.x;
-    if (threadIdx.x == 0) {
-        st_ag<T>(&part[2 * blockIdx.x], ns);
-        st_ag<T>(&part[2 * blockIdx.x + 1], nc);
+    if (NORM) part_out<T, COMP>(ns, nc, part, blockIdx.x);
+}
+
+// ||w|| from G partials, then the Givens step for column j with h_{j+1,j} = ||w||
+template <typename T, bool COMP>
+__global__ __launch_bounds__(DB) void k_gm_nfin(const T* __restrict__ part, unsigned G, int j, int m,
+                                                double* __restrict__ H, int ldh, double* __restrict__ cs,
+                                                double* __restrict__ sn, double* __restrict__ g, T* __restrict__ sig,
+                                                dstate* __restrict__ st, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    double nn[1];
+    merge_columns<T, COMP, 1>(part, G, nn);
+    if (threadIdx.x == 0) givens_step<T>(j, m, sqrt(nn[0]), H, ldh, cs, sn, g, sig, st);
+}
+
+// Selective reorthogonalisation after the first classical pass (Daniel, Gragg,
+// Kaufman and Stewart 1976): w' = w - V h is accepted when ||w'|| >= ||w|| / sqrt 2,
+// and the Givens step runs with h_{j+1,j} = ||w'||; otherwise the second pass runs
+// (skip1 = 0) and k_gm_nfin takes the step after it.
+template <typename T, bool COMP>
+__global__ __launch_bounds__(DB) void k_gm_dgks(const T* __restrict__ part, unsigned G, int j, int m,
+                                                double* __restrict__ H, int ldh, double* __restrict__ cs,
+                                                double* __restrict__ sn, double* __restrict__ g, T* __restrict__ sig,
+                                                dstate* __restrict__ st)
+{
+    if (st->stop) {
+        if (threadIdx.x == 0) st->skip1 = 1;
+        return;
     }
-    if (!last_block(ticket, G)) return;
-    T fs = T(0), fc = T(0);
-    for (unsigned b = threadIdx.x; b < G; b += DB) {
-        const T ps = ld_ag<T>(&part[2 * b]);
-        const T pc = ld_ag<T>(&part[2 * b + 1]);
-        if (COMP) dev::merge<T>(fs, fc, ps, pc);
-        else fs = fs + ps;
-    }
-    dev::block_reduce<T, COMP, DB>(fs, fc);
-    if (threadIdx.x == 0) {
-        givens_step<T>(j, m, sqrt((double)fs + (double)fc), H, ldh, cs, sn, g, sig, st);
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double nn[1];
+    merge_columns<T, COMP, 1>(part, G, nn);
+    if (threadIdx.x != 0) return;
+    if (nn[0] >= 0.5 * st->wn2) {
+        st->skip1 = 1;
+        givens_step<T>(j, m, sqrt(nn[0]), H, ldh, cs, sn, g, sig, st);
+    } else {
+        st->skip1 = 0;
     }
 }
 
@@ -596,38 +649,8 @@ __global__ void k_bs_check(dstate* st, const T* n4, double* hist)
     monitor_check(st, hist, sqrt(dval(n4)));
 }
 
-// ---- fused BiCGStab(L) cycle: reductions finish with their scalar step --------
-
-// Finish NQ reductions: per-block totals to part, the block taking the last ticket
-// merges them in block order; true in thread 0 of that block, with out[q] = s + c.
-template <typename T, int NQ, bool COMP>
-__device__ bool reduce_finish(T (&s)[NQ], T (&c)[NQ], T* part, unsigned* ticket, double (&out)[NQ])
-{
-    const unsigned G = gridDim.x;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        dev::block_reduce<T, COMP, DB>(s[q], c[q]);
-        if (threadIdx.x == 0) {
-            st_ag<T>(&part[2 * ((size_t)q * G + blockIdx.x)], s[q]);
-            st_ag<T>(&part[2 * ((size_t)q * G + blockIdx.x) + 1], c[q]);
-        }
-    }
-    if (!last_block(ticket, G)) return false;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        T fs = T(0), fc = T(0);
-        for (unsigned b = threadIdx.x; b < G; b += DB) {
-            const T ps = ld_ag<T>(&part[2 * ((size_t)q * G + b)]);
-            const T pc = ld_ag<T>(&part[2 * ((size_t)q * G + b) + 1]);
-            if (COMP) dev::merge<T>(fs, fc, ps, pc);
-            else fs = fs + ps;
-        }
-        dev::block_reduce<T, COMP, DB>(fs, fc);
-        if (threadIdx.x == 0) out[q] = (double)fs + (double)fc;
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return threadIdx.x == 0;
-}
+// ---- fused BiCGStab(L) cycle: dots ride on the kernels that produce their operands,
+// a one-block kernel merges them and runs the scalar step
 
 // scalar steps of bicgstabl.h:199-210 on merged dot products
 template <typename T>
@@ -684,20 +707,25 @@ __device__ void epi_cycle(int kind, dstate* st, double d0, double d1, T* coef, d
     epi_beta<T>(st, d0, 1, coef);
 }
 
+// number of dot products a fused kernel hands to k_ds_fin for step EPI
+template <int EPI> struct epi_terms {
+    static constexpr int value = EPI == EPI_NONE ? 0 : (EPI == EPI_ALPHA || EPI == EPI_BETA ? 1 : 2);
+};
+
 // y = A x (MODE 0) or y = b - A x (MODE 1), LPR lanes per row (Dot2 rows when
-// CROWS), with the dot products of y that the cycle needs next:
+// CROWS), with partials of the dot products of y that the cycle needs next
+// (k_ds_fin then runs the step):
 //   EPI_ALPHA  (y, r~) -> alpha;  EPI_BETA (y, r~) -> beta of the next BiCG step;
 //   EPI_CHECK / EPI_REL (y, r~), (y, y) -> monitor check and beta of the next cycle.
+
 template <typename T, int LPR, bool CROWS, bool COMP, int MODE, int EPI>
 __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict__ rp, const int* __restrict__ col,
                                                 const T* __restrict__ val, const T* __restrict__ x,
                                                 const T* __restrict__ b, T* __restrict__ y, const T* __restrict__ rt,
-                                                T* __restrict__ part, unsigned* __restrict__ ticket,
-                                                dstate* __restrict__ st, T* __restrict__ coef, double* hist,
-                                                double delta, const int* __restrict__ skip)
+                                                T* __restrict__ part, const int* __restrict__ skip)
 {
     if (*skip) return;
-    constexpr int NQ = EPI == EPI_NONE ? 0 : (EPI == EPI_ALPHA || EPI == EPI_BETA ? 1 : 2);
+    constexpr int NQ = epi_terms<EPI>::value;
     T ds[NQ > 0 ? NQ : 1], dc[NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) ds[q] = dc[q] = T(0);
@@ -705,6 +733,7 @@ __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict_
     const long row_stride = (long)gridDim.x * (DB / LPR);
     for (long r = ((long)blockIdx.x * DB + threadIdx.x) / LPR; r < rows; r += row_stride) {
         const int beg = rp[r], end = rp[r + 1];
+        const T rtr = (NQ >= 1 && lane == 0) ? rt[r] : T(0);   // issued with the row bounds
         T s = T(0), c = T(0);
         if (MODE == 1 && lane == 0) s = b[r];
         for (int k = beg + lane; k < end; k += LPR) {
@@ -726,8 +755,8 @@ __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict_
             const T v = CROWS ? s + c : s;
             y[r] = v;
             if constexpr (NQ >= 1) {
-                if (COMP) dev::dot2_step<T>(ds[0], dc[0], v, rt[r]);
-                else ds[0] = dev::fma_rn<T>(v, rt[r], ds[0]);
+                if (COMP) dev::dot2_step<T>(ds[0], dc[0], v, rtr);
+                else ds[0] = dev::fma_rn<T>(v, rtr, ds[0]);
             }
             if constexpr (NQ >= 2) {
                 if (COMP) dev::dot2_step<T>(ds[1], dc[1], v, v);
@@ -736,20 +765,32 @@ __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict_
         }
     }
     if constexpr (NQ > 0) {
-        double out[NQ];
-        if (!reduce_finish<T, NQ, COMP>(ds, dc, part, ticket, out)) return;
-        if (EPI == EPI_ALPHA) epi_alpha<T>(st, out[0], coef);
-        else if (EPI == EPI_BETA) epi_beta<T>(st, out[0], 0, coef);
-        else epi_cycle<T>(EPI, st, out[0], out[NQ - 1], coef, hist, delta);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) part_out<T, COMP>(ds[q], dc[q], part, (size_t)q * gridDim.x + blockIdx.x);
     }
 }
 
-// MR part in one pass: (r_0, r_k) and (r_i, r_k), 1 <= i <= k <= L, then the
-// normal equations by Cholesky in FP64 (as k_bs_mr); L <= 4.
+// merged dot products of a fused kernel (G partials per term), then its scalar step
+template <typename T, bool COMP, int EPI>
+__global__ __launch_bounds__(DB) void k_ds_fin(const T* __restrict__ part, unsigned G, dstate* __restrict__ st,
+                                               T* __restrict__ coef, double* hist, double delta,
+                                               const int* __restrict__ skip)
+{
+    if (*skip) return;
+    constexpr int NQ = epi_terms<EPI>::value;
+    double out[NQ];
+    merge_columns<T, COMP, NQ>(part, G, out);
+    if (threadIdx.x != 0) return;
+    if (EPI == EPI_ALPHA) epi_alpha<T>(st, out[0], coef);
+    else if (EPI == EPI_BETA) epi_beta<T>(st, out[0], 0, coef);
+    else epi_cycle<T>(EPI, st, out[0], out[NQ - 1], coef, hist, delta);
+}
+
+// MR part in one pass: partials of (r_0, r_k) and (r_i, r_k), 1 <= i <= k <= L;
+// k_ds_gfin solves the normal equations by Cholesky in FP64 (as k_bs_mr); L <= 4.
 template <typename T, int L, bool COMP>
 __global__ __launch_bounds__(DB) void k_ds_gram(size_t n, bs_pack<T> P, T* __restrict__ part,
-                                                unsigned* __restrict__ ticket, dstate* __restrict__ st,
-                                                T* __restrict__ coef, const int* __restrict__ skip)
+                                                const int* __restrict__ skip)
 {
     if (*skip) return;
     constexpr int NP = L + L * (L + 1) / 2;
@@ -771,8 +812,20 @@ __global__ __launch_bounds__(DB) void k_ds_gram(size_t n, bs_pack<T> P, T* __res
             }
         }
     }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) part_out<T, COMP>(s[q], c[q], part, (size_t)q * gridDim.x + blockIdx.x);
+}
+
+// the Gram matrix and right-hand side from Gp partials per entry, then Cholesky
+template <typename T, int L, bool COMP>
+__global__ __launch_bounds__(DB) void k_ds_gfin(const T* __restrict__ part, unsigned Gp, dstate* __restrict__ st,
+                                                T* __restrict__ coef, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    constexpr int NP = L + L * (L + 1) / 2;
     double out[NP];
-    if (!reduce_finish<T, NP, COMP>(s, c, part, ticket, out)) return;
+    merge_columns<T, COMP, NP>(part, Gp, out);
+    if (threadIdx.x != 0) return;
     double G[L][L], cv[L], yv[L];
     int q = 0;
     for (int k = 1; k <= L; ++k)
@@ -813,14 +866,13 @@ __global__ __launch_bounds__(DB) void k_ds_gram(size_t n, bs_pack<T> P, T* __res
     for (int j = 0; j < L; ++j) coef[2 + j] = (T)yv[j];
 }
 
-// MR update of x, r_0, u_0 (as k_bs_upd_mr) with (r_0, r~), (r_0, r_0) of the new
-// r_0 and the end-of-cycle step (monitor check or reliable-update trigger, beta)
-template <typename T, bool COMP, int EPI>
+// MR update of x, r_0, u_0 (as k_bs_upd_mr) with partials of (r_0, r~), (r_0, r_0)
+// of the new r_0; k_ds_fin runs the end-of-cycle step (monitor check or
+// reliable-update trigger, beta)
+template <typename T, bool COMP>
 __global__ __launch_bounds__(DB) void k_ds_upd_mr(size_t n, int L, bs_pack<T> P, T* __restrict__ x,
-                                                  const T* __restrict__ rt, T* __restrict__ coef,
-                                                  T* __restrict__ part, unsigned* __restrict__ ticket,
-                                                  dstate* __restrict__ st, double* hist, double delta,
-                                                  const int* __restrict__ skip)
+                                                  const T* __restrict__ rt, const T* __restrict__ coef,
+                                                  T* __restrict__ part, const int* __restrict__ skip)
 {
     if (*skip) return;
     __shared__ T gm[MAXL];
@@ -857,9 +909,8 @@ __global__ __launch_bounds__(DB) void k_ds_upd_mr(size_t n, int L, bs_pack<T> P,
             ds[1] = dev::fma_rn<T>(r0, r0, ds[1]);
         }
     }
-    double out[2];
-    if (!reduce_finish<T, 2, COMP>(ds, dc, part, ticket, out)) return;
-    epi_cycle<T>(EPI, st, out[0], out[1], coef, hist, delta);
+    part_out<T, COMP>(ds[0], dc[0], part, blockIdx.x);
+    part_out<T, COMP>(ds[1], dc[1], part, (size_t)gridDim.x + blockIdx.x);
 }
 
 // ---- reliable updates (FP32 iterations, FP64 solution and true residual)
@@ -971,13 +1022,17 @@ struct graph_guard {
     }
 };
 
-unsigned vgrid(const hpb_handle_s* h, size_t n)
+// grid of the streaming vector kernels: up to 8 blocks per CU
+unsigned vgrid(const hpb_handle_s* h, size_t n, size_t per_cu = 8)
 {
     size_t g = (n + DB - 1) / DB;
-    const size_t cap = (size_t)(h->num_cu > 0 ? h->num_cu : 256) * 8;
+    const size_t cap = (size_t)(h->num_cu > 0 ? h->num_cu : 256) * per_cu;
     if (g > cap) g = cap;
     return (unsigned)(g < 1 ? 1 : g);
 }
+// vector kernels that also leave per-block partials: 4 blocks per CU stream as
+// fast and halve what the finishing kernel merges
+unsigned rgrid(const hpb_handle_s* h, size_t n) { return vgrid(h, n, 4); }
 
 #define DS_CHECK(call)                  \
     do {                                \
@@ -1101,7 +1156,8 @@ public:
                                   pr.ci == P.ci && pr.val == P.val && pr.b == P.b && pr.x == P.x;
         const bool same_params = p.solver == prm.solver && p.restart == prm.restart && p.basis == prm.basis &&
                                  p.high_precision == prm.high_precision &&
-                                 p.compensated_spmv == prm.compensated_spmv;
+                                 p.compensated_spmv == prm.compensated_spmv && p.gram_schmidt == prm.gram_schmidt &&
+                                 p.resid_recalc_freq == prm.resid_recalc_freq;
         return same_problem && same_params && hist_cap == hcap;
     }
     // monitor settings (tolerances, iteration caps, early exit, recalculation)
@@ -1140,6 +1196,7 @@ private:
         const size_t n = (size_t)P.rows;
         const int m = prm.restart;
         const bool comp = prm.high_precision != 0, cspmv = prm.compensated_spmv != 0;
+        const bool dgks = prm.gram_schmidt == 2;   // else CGS2: both passes always
         const int ldh = m + 1;
         Vb.resize(m + 1);
         V.assign(m + 1, nullptr);
@@ -1163,11 +1220,12 @@ private:
         double* hp = hist();
         T* sig = scal.p;
         T* part = reinterpret_cast<T*>(h->part);
-        unsigned* tk = h->ticket + 3;
         const size_t cus = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
         const size_t part_pairs = h->part_bytes / (2 * sizeof(T));
-        const size_t gfull = (n + DB - 1) / DB;
-        const unsigned gupd = (unsigned)std::max<size_t>(1, std::min(gfull, 8 * cus));
+        // grids over 16-byte vectors; the reductions end in per-block partials
+        // merged by one-block kernels (k_gm_hfin, k_gm_nfin)
+        const size_t gvec = (n / dev::vec16<T>::W + DB - 1) / DB;
+        const unsigned gupd = (unsigned)std::max<size_t>(1, std::min(gvec, 8 * cus));
         constexpr int KC = 8;
         HPB_HIP(hipStreamSynchronize(st));
         HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1180,29 +1238,44 @@ private:
                 // w = A (sig_j V_j), stored unnormalised in V[j+1]
                 DS_CHECK(csr_apply<T>(h, P.rows, P.nnz, P.rp, P.ci, P.val, V[j], nullptr, V[j + 1], cspmv, stop,
                                       sig + j));
-                const unsigned nch = (unsigned)((k + KC - 1) / KC);
-                size_t gd = std::min(gfull, std::max<size_t>(cus / 2, (8 * cus) / nch));
-                gd = std::max<size_t>(1, std::min(gd, part_pairs / (size_t)k));
                 double* Hj = H.p + (size_t)j * ldh;
                 for (int pass = 0; pass < 2; ++pass) {
-                    if (comp)
-                        hipLaunchKernelGGL((k_gm_dot<T, KC, true>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, k, bp,
-                                           V[j + 1], sig, part, tk, Hj, coef.p, pass, stop);
-                    else
-                        hipLaunchKernelGGL((k_gm_dot<T, KC, false>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, k,
-                                           bp, V[j + 1], sig, part, tk, Hj, coef.p, pass, stop);
+                    // pass 1 runs under skip1 when the first pass may be accepted
+                    const int* skip = (dgks && pass == 1) ? &S->skip1 : stop;
+                    const int kk = (dgks && pass == 0) ? k + 1 : k;   // + (w, w)
+                    const unsigned nch = (unsigned)((kk + KC - 1) / KC);
+                    size_t gd = std::min(gvec, std::max<size_t>(cus, (4 * cus) / nch));
+                    gd = std::max<size_t>(1, std::min(gd, part_pairs / (size_t)kk));
+                    if (comp) {
+                        hipLaunchKernelGGL((k_gm_dot<T, KC, true>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, kk,
+                                           bp, V[j + 1], part, skip);
+                        hipLaunchKernelGGL((k_gm_hfin<T, true>), dim3(1), dim3(1024), 0, st, part, (unsigned)gd, k,
+                                           kk, sig, Hj, coef.p, pass, skip, S);
+                    } else {
+                        hipLaunchKernelGGL((k_gm_dot<T, KC, false>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, kk,
+                                           bp, V[j + 1], part, skip);
+                        hipLaunchKernelGGL((k_gm_hfin<T, false>), dim3(1), dim3(1024), 0, st, part, (unsigned)gd, k,
+                                           kk, sig, Hj, coef.p, pass, skip, S);
+                    }
                     auto upd = [&](auto compc, auto normc) {
-                        hipLaunchKernelGGL((k_gm_upd<T, decltype(compc)::value, decltype(normc)::value>), dim3(gupd),
-                                           dim3(DB), 0, st, n, k, bp, coef.p, V[j + 1], part, tk, j, m, H.p, ldh,
-                                           cs.p, sn.p, g.p, sig, S);
+                        constexpr bool C = decltype(compc)::value, NM = decltype(normc)::value;
+                        hipLaunchKernelGGL((k_gm_upd<T, C, NM>), dim3(gupd), dim3(DB), 0, st, n, k, bp, coef.p,
+                                           V[j + 1], part, skip);
+                        if (NM && dgks && pass == 0)
+                            hipLaunchKernelGGL((k_gm_dgks<T, C>), dim3(1), dim3(DB), 0, st, part, gupd, j, m, H.p,
+                                               ldh, cs.p, sn.p, g.p, sig, S);
+                        else if (NM)
+                            hipLaunchKernelGGL((k_gm_nfin<T, C>), dim3(1), dim3(DB), 0, st, part, gupd, j, m, H.p,
+                                               ldh, cs.p, sn.p, g.p, sig, S, skip);
                     };
                     using BT = std::true_type;
                     using BF = std::false_type;
+                    const bool norm = pass == 1 || dgks;
                     if (comp) {
-                        if (pass) upd(BT{}, BT{});
+                        if (norm) upd(BT{}, BT{});
                         else upd(BT{}, BF{});
                     } else {
-                        if (pass) upd(BF{}, BT{});
+                        if (norm) upd(BF{}, BT{});
                         else upd(BF{}, BF{});
                     }
                 }
@@ -1243,17 +1316,16 @@ private:
     {
         hpb_handle_t h = P.h;
         const double mean = P.rows > 0 ? (double)P.nnz / P.rows : 1.0;
-        const int lpr = mean <= 6.0 ? 4 : (mean <= 12.0 ? 8 : 16);
+        const int lpr = mean <= 12.0 ? 4 : (mean <= 24.0 ? 8 : 16);   // as the library SpMV
         const long per_block = DB / lpr;
         long grid = (P.rows + per_block - 1) / per_block;
-        const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;
+        const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;   // the gathers want many waves
         grid = std::max(1L, std::min(grid, cap));
         const bool cr = prm.compensated_spmv != 0, comp = prm.high_precision != 0;
         T* part = reinterpret_cast<T*>(h->part);
-        unsigned* tk = h->ticket + 3;
 #define DS_SPMV(L_, CR_, CO_)                                                                                \
     hipLaunchKernelGGL((k_ds_spmv<T, L_, CR_, CO_, MODE, EPI>), dim3((unsigned)grid), dim3(DB), 0, h->stream,   \
-                       P.rows, P.rp, P.ci, P.val, x, b, y, rtv, part, tk, dst.p, coef.p, hist(), R.delta, skip)
+                       P.rows, P.rp, P.ci, P.val, x, b, y, rtv, part, skip)
 #define DS_SPMV_L(L_)                                                                                        \
     do {                                                                                                     \
         if (cr) { if (comp) DS_SPMV(L_, true, true); else DS_SPMV(L_, true, false); }                        \
@@ -1264,6 +1336,21 @@ private:
         else DS_SPMV_L(16);
 #undef DS_SPMV_L
 #undef DS_SPMV
+        if constexpr (EPI != EPI_NONE) ds_fin<EPI>((unsigned)grid, skip);
+    }
+
+    // the scalar step after a fused kernel whose G blocks left partials in part
+    template <int EPI>
+    void ds_fin(unsigned G, const int* skip)
+    {
+        hpb_handle_t h = P.h;
+        const T* part = reinterpret_cast<const T*>(h->part);
+        if (prm.high_precision)
+            hipLaunchKernelGGL((k_ds_fin<T, true, EPI>), dim3(1), dim3(DB), 0, h->stream, part, G, dst.p, coef.p,
+                               hist(), R.delta, skip);
+        else
+            hipLaunchKernelGGL((k_ds_fin<T, false, EPI>), dim3(1), dim3(DB), 0, h->stream, part, G, dst.p, coef.p,
+                               hist(), R.delta, skip);
     }
 
     bool fused_ok() const
@@ -1277,12 +1364,16 @@ private:
         hpb_handle_t h = P.h;
         const size_t n = (size_t)P.rows;
         T* part = reinterpret_cast<T*>(h->part);
-        if (prm.high_precision)
-            hipLaunchKernelGGL((k_ds_gram<T, L, true>), dim3(vgrid(h, n)), dim3(DB), 0, h->stream, n, bp, part,
-                               h->ticket + 3, dst.p, coef.p, skip);
-        else
-            hipLaunchKernelGGL((k_ds_gram<T, L, false>), dim3(vgrid(h, n)), dim3(DB), 0, h->stream, n, bp, part,
-                               h->ticket + 3, dst.p, coef.p, skip);
+        const unsigned G = rgrid(h, n);
+        if (prm.high_precision) {
+            hipLaunchKernelGGL((k_ds_gram<T, L, true>), dim3(G), dim3(DB), 0, h->stream, n, bp, part, skip);
+            hipLaunchKernelGGL((k_ds_gfin<T, L, true>), dim3(1), dim3(DB), 0, h->stream, part, G, dst.p, coef.p,
+                               skip);
+        } else {
+            hipLaunchKernelGGL((k_ds_gram<T, L, false>), dim3(G), dim3(DB), 0, h->stream, n, bp, part, skip);
+            hipLaunchKernelGGL((k_ds_gfin<T, L, false>), dim3(1), dim3(DB), 0, h->stream, part, G, dst.p, coef.p,
+                               skip);
+        }
     }
 
     int setup_bicgstabl()
@@ -1409,22 +1500,15 @@ private:
                     hipLaunchKernelGGL((k_bs_upd_mr<T, false>), dim3(vg), dim3(DB), 0, st, n, L, bp, P.x, coef.p, skip);
                 ds_spmv<1, EPI_CHECK>(P.x, P.b, V[0], rt.p, skip);
             } else {
-                auto upd = [&](auto compc, auto epic) {
-                    hipLaunchKernelGGL((k_ds_upd_mr<T, decltype(compc)::value, decltype(epic)::value>), dim3(vg),
-                                       dim3(DB), 0, st, n, L, bp, P.x, rt.p, coef.p, part, h->ticket + 3, S, hist(),
-                                       R.delta, skip);
-                };
-                using CT = std::true_type;
-                using CF = std::false_type;
-                using EC = std::integral_constant<int, EPI_CHECK>;
-                using ER = std::integral_constant<int, EPI_REL>;
-                if (comp) {
-                    if (epi == EPI_REL) upd(CT{}, ER{});
-                    else upd(CT{}, EC{});
-                } else {
-                    if (epi == EPI_REL) upd(CF{}, ER{});
-                    else upd(CF{}, EC{});
-                }
+                const unsigned rg = rgrid(h, n);
+                if (comp)
+                    hipLaunchKernelGGL((k_ds_upd_mr<T, true>), dim3(rg), dim3(DB), 0, st, n, L, bp, P.x, rt.p, coef.p,
+                                       part, skip);
+                else
+                    hipLaunchKernelGGL((k_ds_upd_mr<T, false>), dim3(rg), dim3(DB), 0, st, n, L, bp, P.x, rt.p,
+                                       coef.p, part, skip);
+                if (epi == EPI_REL) ds_fin<EPI_REL>(rg, skip);
+                else ds_fin<EPI_CHECK>(rg, skip);
             }
             return HPB_OK;
         }();

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(XBLOCK) void k_exact(size_t n, const void* __restri
     if (threadIdx.x == 0) st_agent_ll(&rec[XL], (long long)sflag);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         am_last = (prev == G - 1);
     }
     __syncthreads();

@@ -10,9 +10,10 @@
 //
 // multidot grid: blockIdx.y picks a chunk of KC columns, blockIdx.x strides over
 // tiles of w; every thread keeps KC (s, c) accumulators.  Per-block partials go
-// to the handle workspace; the block taking the last ticket merges each column's
-// G partials in block order (one wave per column: lane l takes partials
-// l, l+64, ...; then the wave butterfly), so results are reproducible.
+// to the handle workspace and a one-block kernel merges each column's G partials
+// in a fixed order.  (No last-block ticket: its agent-scope release in every
+// block costs an L2 write-back each on a multi-XCD part, more than the second
+// launch for grids of more than a few hundred blocks.)
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -31,24 +32,9 @@ struct coef_pack {
     T a[HPB_MAX_MULTIDOT];
 };
 
-template <typename T> struct ubits { using type = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned>; };
-template <typename T>
-__device__ __forceinline__ void st_ag(T* p, T v)
-{
-    using U = typename ubits<T>::type;
-    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T ld_ag(const T* p)
-{
-    using U = typename ubits<T>::type;
-    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
 template <typename T, int KC, bool COMP, int BLOCK, bool VEC>
 __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T> V, const T* __restrict__ w,
-                                                    T* __restrict__ part, unsigned* __restrict__ ticket,
-                                                    T* __restrict__ out, const int* __restrict__ skip)
+                                                    T* __restrict__ part, const int* __restrict__ skip)
 {
     if (skip != nullptr && *skip) return;   // solver graphs: flag set, launch does nothing
     using Vt = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
@@ -105,27 +91,27 @@ __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T>
             T ss = s[j], cc = c[j];
             dev::block_reduce<T, COMP, BLOCK>(ss, cc);
             if (threadIdx.x == 0) {
-                st_ag<T>(&part[2 * ((size_t)(j0 + j) * G + blockIdx.x)], ss);
-                st_ag<T>(&part[2 * ((size_t)(j0 + j) * G + blockIdx.x) + 1], cc);
+                part[2 * ((size_t)(j0 + j) * G + blockIdx.x)] = ss;
+                part[2 * ((size_t)(j0 + j) * G + blockIdx.x) + 1] = cc;
             }
         }
     }
-    __shared__ int am_last;
-    if (threadIdx.x == 0) {
-        const unsigned total = G * gridDim.y;
-        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == total - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Merge of the G partials of each column, one wave per column (lane l takes
+// partials l, l+64, ...; then the wave butterfly): a fixed order, so results
+// are reproducible.  out[4j..4j+3] = { s, c, s + c, s + c } (plain: c = 0).
+template <typename T, bool COMP>
+__global__ __launch_bounds__(1024) void k_multidot_fin(const T* __restrict__ part, unsigned G, int k,
+                                                       T* __restrict__ out, const int* __restrict__ skip)
+{
+    if (skip != nullptr && *skip) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    constexpr int NW = BLOCK / 64;
-    for (int j = wid; j < k; j += NW) {
+    for (int j = wid; j < k; j += 1024 / 64) {
         T fs = T(0), fc = T(0);
         for (unsigned b = lane; b < G; b += 64) {
-            const T ps = ld_ag<T>(&part[2 * ((size_t)j * G + b)]);
-            const T pc = ld_ag<T>(&part[2 * ((size_t)j * G + b) + 1]);
+            const T ps = part[2 * ((size_t)j * G + b)];
+            const T pc = part[2 * ((size_t)j * G + b) + 1];
             if (COMP) dev::merge<T>(fs, fc, ps, pc);
             else fs = fs + ps;
         }
@@ -138,8 +124,6 @@ __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T>
             out[4 * j + 3] = COMP ? r : fs;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // y := y + sum_j a_j V_j (compensated: Dot2 over the k+1 terms of each element)
@@ -220,25 +204,26 @@ static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, co
     const int nch = (k + KC - 1) / KC;
     const size_t W = vec ? dev::vec16<T>::W : 1;
     const size_t nv = n / W;
-    // about 8 blocks per CU in total over the column chunks: one chunk alone
-    // needs several blocks per CU to keep enough loads in flight
+    // about 4 blocks per CU in total over the column chunks (at least one per
+    // CU per chunk): enough loads in flight, few partials to merge
     size_t g = (nv + 255) / 256;
     const size_t cus = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
-    size_t cap = (8 * cus) / (size_t)nch;
-    if (cap < cus / 2) cap = cus / 2;
+    size_t cap = (4 * cus) / (size_t)nch;
+    if (cap < cus) cap = cus;
     if (cap * (size_t)k > (size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) cap = ((size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) / k;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
     T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
     T* part = reinterpret_cast<T*>(h->part);
-    unsigned* tk = h->ticket + 1;
     dim3 grid((unsigned)g, (unsigned)nch);
     if (comp) {
-        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, true, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
-        else hipLaunchKernelGGL((k_multidot<T, KC, true, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
+        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, true, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
+        else hipLaunchKernelGGL((k_multidot<T, KC, true, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
+        hipLaunchKernelGGL((k_multidot_fin<T, true>), dim3(1), dim3(1024), 0, h->stream, part, (unsigned)g, k, out, skip);
     } else {
-        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, false, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
-        else hipLaunchKernelGGL((k_multidot<T, KC, false, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, tk, out, skip);
+        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, false, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
+        else hipLaunchKernelGGL((k_multidot<T, KC, false, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
+        hipLaunchKernelGGL((k_multidot_fin<T, false>), dim3(1), dim3(1024), 0, h->stream, part, (unsigned)g, k, out, skip);
     }
     HPB_LAUNCH_CHECK();
     if (!dev_out) {

@@ -102,12 +102,16 @@ __device__ __forceinline__ void final_merge(const T* part, unsigned G, T& fs, T&
     dev::block_reduce<T, COMP, BLOCK>(fs, fc);
 }
 
-// Returns true in every thread of the block that took the last ticket.
+// Returns true in every thread of the block that took the last ticket.  Every
+// block publishes its partial with a release increment (an L2 write-back on a
+// multi-XCD part); only the last block needs the acquire, taken as a fence after
+// it has seen the final count, so the other blocks skip the L2 invalidate that an
+// acq_rel increment would cost each of them.
 __device__ __forceinline__ bool take_ticket(unsigned* ticket, unsigned G)
 {
     __shared__ int am_last;
     if (threadIdx.x == 0) {
-        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         am_last = (prev == G - 1);
     }
     __syncthreads();

@@ -92,7 +92,9 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
         if (prec) s.set_preconditioner(prec);
         s.set_use_precond_resid(p.use_precond_resid != 0);
         s.set_restarts((unsigned)p.restart);
-        s.set_gram_schmidt(p.gram_schmidt == 1 ? gram_schmidt::cgs2 : gram_schmidt::mgs);
+        s.set_gram_schmidt(p.gram_schmidt == 1   ? gram_schmidt::cgs2
+                           : p.gram_schmidt == 2 ? gram_schmidt::cgs_dgks
+                                                 : gram_schmidt::mgs);
         s.set_early_exit(p.early_exit != 0);
         s.set_reorthogonalization(p.reorthogonalization != 0);
         s.monitor().init((T)rel_tol, (T)p.abs_tol, p.max_iters, p.min_iters, false, p.save_history != 0, true);
@@ -126,7 +128,7 @@ bool run_solver(const hpb_solver_params& p, hpb::log_std* log, const typename so
 bool use_device(const hpb_solver_params& p)
 {
     if (!p.device_resident || p.preconditioner != HPB_PRECOND_NONE) return false;
-    if (p.solver == HPB_SOLVER_GMRES) return p.gram_schmidt == 1 && !p.reorthogonalization;
+    if (p.solver == HPB_SOLVER_GMRES) return (p.gram_schmidt == 1 || p.gram_schmidt == 2) && !p.reorthogonalization;
     if (p.solver == HPB_SOLVER_BICGSTAB) return false;
     return p.basis <= 8;
 }
@@ -137,6 +139,7 @@ int validate(const hpb_solver_params* p)
     if (p->solver != HPB_SOLVER_GMRES && p->solver != HPB_SOLVER_BICGSTABL && p->solver != HPB_SOLVER_BICGSTAB)
         return HPB_ERR_INVALID;
     if (p->solver == HPB_SOLVER_GMRES && (p->restart < 1 || p->restart >= 256)) return HPB_ERR_INVALID;
+    if (p->gram_schmidt < 0 || p->gram_schmidt > 2) return HPB_ERR_INVALID;
     if (p->solver == HPB_SOLVER_BICGSTABL && (p->basis < 1 || p->basis > 30)) return HPB_ERR_INVALID;
     if (p->max_iters < 1 || !(p->rel_tol >= 0)) return HPB_ERR_INVALID;
     if (p->preconditioner != HPB_PRECOND_NONE && p->preconditioner != HPB_PRECOND_ILU0) return HPB_ERR_INVALID;

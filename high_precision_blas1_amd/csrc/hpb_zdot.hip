@@ -110,7 +110,7 @@ __global__ __launch_bounds__(BLOCK) void k_zdot(size_t n, const T* __restrict__ 
     }
     __shared__ int am_last;
     if (tid == 0) {
-        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         am_last = (prev == G - 1);
     }
     __syncthreads();

@@ -16,6 +16,9 @@ BICGSTABL = 1
 BICGSTAB = 2
 PRECOND_NONE = 0
 PRECOND_ILU0 = 1
+GS_MGS = 0       # gram_schmidt: the reference's modified Gram-Schmidt
+GS_CGS2 = 1      # two classical passes
+GS_DGKS = 2      # classical, second pass only when the first shrank w by more than 1/sqrt(2)
 
 
 class SolverParams(ctypes.Structure):

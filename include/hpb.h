@@ -348,7 +348,9 @@ typedef struct hpb_solver_params {
     double rel_tol, abs_tol; /* monitor: ||r|| <= abs_tol + rel_tol ||b|| (default_monitor.h:159) */
     int high_precision;      /* 1: compensated Dot2 / nrm2 in the solver (use_high_precision) */
     int compensated_spmv;    /* 1: compensated-row SpMV inside the solver */
-    int gram_schmidt;        /* GMRES: 0 MGS (the reference's), 1 CGS2 with fused kernels */
+    int gram_schmidt;        /* GMRES: 0 MGS (the reference's), 1 CGS2 with fused kernels, 2 CGS with
+                                the second pass only when the first shrank w by more than 1/sqrt(2)
+                                (Daniel-Gragg-Kaufman-Stewart selective reorthogonalisation) */
     int early_exit;          /* GMRES: leave a cycle when the Givens estimate meets the tolerance */
     int reorthogonalization; /* GMRES MGS: one extra correction per basis vector */
     int resid_recalc_freq;   /* BiCGStab(L) / BiCGStab: recompute b - Ax every k cycles / steps (0 never) */

@@ -104,17 +104,19 @@ static void run_all(const char* tag, bool high)
     vo.start_use_vector(x);
     for (int i = 0; i < n; ++i) b[i] = bh[i];
     char name[128];
-    for (int gs = 0; gs < 2; ++gs)
+    const gram_schmidt variants[3] = {gram_schmidt::mgs, gram_schmidt::cgs2, gram_schmidt::cgs_dgks};
+    const char* vname[3] = {"mgs", "cgs2", "dgks"};
+    for (int gs = 0; gs < 3; ++gs)
         for (int pc = 0; pc < 2; ++pc) {
             gmres<op_t, prec_t, VO, mon_t, hpb::log_std, 100> s(&vo);
             prec_t P;
             if (pc) s.set_preconditioner(&P);
             s.set_restarts(30);
-            s.set_gram_schmidt(gs ? gram_schmidt::cgs2 : gram_schmidt::mgs);
+            s.set_gram_schmidt(variants[gs]);
             s.monitor().init(1e-10, 0.0, 5000);
             vo.assign_scalar(0.0, x);
             bool ok = s.solve(A, b, x);
-            std::snprintf(name, sizeof name, "%s_gmres_%s_prec%d", tag, gs ? "cgs2" : "mgs", pc);
+            std::snprintf(name, sizeof name, "%s_gmres_%s_prec%d", tag, vname[gs], pc);
             report(name, vo, A, ok, s.monitor().iters_performed(), x, b, xd);
         }
     for (int L : {1, 2, 4}) {

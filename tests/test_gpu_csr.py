@@ -119,6 +119,15 @@ def test_diffusion2d_builder(handle):
     assert abs(g.mean()) < 0.01 and abs(g.std() - 1.0) < 0.01
 
 
+def default_lpr(M):
+    """The library's lanes per row for lpr = 0: about two entries per lane."""
+    mean = M.nnz / M.shape[0]
+    for lim, lanes in ((1.5, 1), (3, 2), (12, 4), (24, 8), (48, 16), (96, 32)):
+        if mean <= lim:
+            return lanes
+    return 64
+
+
 @pytest.mark.parametrize("lpr", [0, 1, 2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("comp", [False, True])
 def test_spmv_bitexact(handle, lpr, comp):
@@ -133,10 +142,7 @@ def test_spmv_bitexact(handle, lpr, comp):
         A = upload_csr(handle, Mh)
         x = r.standard_normal(Mh.shape[0])
         dx = H.DeviceArray.from_numpy(x, handle)
-        eff = lpr if lpr else {True: 4, False: 4}[True]
-        if lpr == 0:
-            mean = Mh.nnz / Mh.shape[0]
-            eff = 1 if mean <= 1.5 else 2 if mean <= 3 else 4 if mean <= 6 else 8 if mean <= 12 else 16
+        eff = lpr if lpr else default_lpr(Mh)
         y = spmv(handle, A, dx, lpr=lpr, comp=comp).numpy()
         assert y.tobytes() == O.csr_spmv(Mh, x, lpr=eff, comp=comp).tobytes()
         y0 = r.standard_normal(Mh.shape[0])
@@ -155,7 +161,7 @@ def test_residual_bitexact(handle):
     out = H.DeviceArray(Mh.shape[0], np.float64, handle)
     for name, comp in (("hpb_csr_residual_f64", False), ("hpb_csr_residualc_f64", True)):
         H.call(name, handle.ptr, A.rows, A.rows, A.nnz, A.rp.ptr, A.col.ptr, A.val.ptr, db.ptr, dx.ptr, out.ptr)
-        assert out.numpy().tobytes() == O.csr_spmv(Mh, x, lpr=8, comp=comp, b=b).tobytes()
+        assert out.numpy().tobytes() == O.csr_spmv(Mh, x, lpr=default_lpr(Mh), comp=comp, b=b).tobytes()
 
 
 def test_poisson_256_cubed_properties(handle):

@@ -29,18 +29,42 @@ def _run(handle, A, b, hist=0, **kw):
     return res, x
 
 
-@pytest.mark.parametrize("m,high,ee", [(30, 1, 1), (30, 0, 1), (20, 1, 0), (7, 1, 1)])
-def test_gmres_device_matches_host(handle, m, high, ee):
+@pytest.mark.parametrize("m,high,ee,gs", [(30, 1, 1, S.GS_CGS2), (30, 0, 1, S.GS_CGS2), (20, 1, 0, S.GS_CGS2),
+                                          (7, 1, 1, S.GS_CGS2), (30, 1, 1, S.GS_DGKS), (20, 1, 0, S.GS_DGKS),
+                                          (7, 0, 1, S.GS_DGKS)])
+def test_gmres_device_matches_host(handle, m, high, ee, gs):
     Mh, A, b = _setup(handle)
-    say("dgmres", m, high, ee)
-    kw = dict(solver=S.GMRES, restart=m, rel_tol=1e-11, max_iters=4000, high_precision=high, gram_schmidt=1,
+    say("dgmres", m, high, ee, gs)
+    kw = dict(solver=S.GMRES, restart=m, rel_tol=1e-11, max_iters=4000, high_precision=high, gram_schmidt=gs,
               early_exit=ee)
     rd, xd = _run(handle, A, b, device_resident=1, **kw)
     rh, xh = _run(handle, A, b, device_resident=0, **kw)
     assert rd["converged"] == 1 and rh["converged"] == 1
-    assert abs(rd["iters"] - rh["iters"]) <= 1, (rd["iters"], rh["iters"])
+    # selective reorthogonalisation: host and device may decide differently on a
+    # step whose ratio ||w'|| / ||w|| sits at 1/sqrt(2) within rounding
+    slack = 1 if gs == S.GS_CGS2 else max(2, rh["iters"] // 50)
+    assert abs(rd["iters"] - rh["iters"]) <= slack, (rd["iters"], rh["iters"])
     assert rd["true_rel_resid"] <= 1.5e-11
     assert np.linalg.norm(xd - xh) <= 1e-8 * np.linalg.norm(xh)
+    A.free()
+
+
+@pytest.mark.parametrize("case", ["convdiff", "aniso"])
+def test_gmres_selective_reorthogonalisation(handle, case):
+    """One classical pass where it keeps orthogonality, two where it would not: the
+    true residual meets the tolerance and the step count stays within 3 % of CGS2's,
+    also on the anisotropic operator (k_z / k_x = 1e4) where CGS1 alone loses
+    orthogonality."""
+    Mh, A, b = _setup(handle, nx=16, coef=S.CONVDIFF7 if case == "convdiff" else S.ANISO7)
+    say("dgks", case)
+    its = {}
+    for gs in (S.GS_CGS2, S.GS_DGKS):
+        r, x = _run(handle, A, b, device_resident=1, solver=S.GMRES, restart=40, rel_tol=1e-12, max_iters=20000,
+                    gram_schmidt=gs)
+        assert r["converged"] == 1 and r["true_rel_resid"] <= 1.2e-12, r
+        assert np.linalg.norm(Mh @ x - b) <= 1.2e-12 * np.linalg.norm(b)
+        its[gs] = r["iters"]
+    assert abs(its[S.GS_DGKS] - its[S.GS_CGS2]) <= max(2, its[S.GS_CGS2] * 3 // 100), its
     A.free()
 
 

@@ -1,7 +1,7 @@
 """The solver templates (cpp/lin_solvers) compiled with g++ against a host test
 double of the VectorOperations concept and run on the reference's 1-D upwind
 convection-diffusion problem (source/linear_operator.hpp:28-90): every variant
-(GMRES MGS/CGS2, BiCGStab(L), plain BiCGStab with and without the
+(GMRES MGS/CGS2/selective CGS, BiCGStab(L), plain BiCGStab with and without the
 preconditioner, preconditioned or plain residual, with and without residual
 recomputation) converges to the tolerance, agrees with a direct tridiagonal
 solve, and the failure paths (iteration cap, NaN input) stop with the documented
@@ -32,7 +32,7 @@ def results(tmp_path_factory):
 
 def test_all_variants_converge(results):
     solves = {k: v for k, v in results.items() if "gmres" in k or "bicgstab" in k}
-    assert len(solves) == 21 + 24
+    assert len(solves) == 21 + 24 + 6
     for name, (conv, iters, rel, err) in solves.items():
         assert conv == "1", name
         assert float(rel) <= 1.5e-10, (name, rel)
@@ -46,7 +46,8 @@ def test_preconditioning_and_restart_iteration_counts(results):
     for tag in ("plain", "high", "multi"):
         a = int(results["%s_gmres_mgs_prec0" % tag][1])
         b = int(results["%s_gmres_cgs2_prec0" % tag][1])
-        assert abs(a - b) <= 30
+        c = int(results["%s_gmres_dgks_prec0" % tag][1])
+        assert abs(a - b) <= 30 and abs(c - b) <= 30
 
 
 def test_failure_paths(results):
