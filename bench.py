@@ -436,8 +436,7 @@ def cfg_mixed(H, h, log):
         {"gmres50_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=50, **G),
          "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
         {"gmres50_fp32_refined": S.params(rel_tol=1e-2, max_iters=5000, restart=50, compensated_spmv=1, **G),
-         "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, compensated_spmv=1,
-                                             **B)})
+         "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, **B)})
     res["aniso256"]["matrix"] = "256^3 7-point anisotropic diffusion, k_z / k_x = 1e4, Dirichlet"
     return res
 
@@ -500,7 +499,12 @@ def cfg_batch(H, h, log, nsys=512):
            rr.ctypes.data_as(ctypes.c_void_p), cv.ctypes.data_as(ctypes.c_void_p), ctypes.byref(secs),
            ctypes.byref(cyc))
     H.call("hpb_batch_destroy", b)
+    # lower bound on the bytes moved: each system's iterations priced with the fused
+    # single-system BiCGStab(2) cycle model (the batch cycle moves at least that much)
+    nbytes = sum(_bicgstabl_bytes(2, m * m, 5 * m * m - 4 * m, 8, 8, (int(k) + 1) // 2, recalc=False)
+                 for m, k in zip(sides, it))
     return {"systems": nsys, "sides_min_max": [min(sides), max(sides)], "rows": rows.value, "nnz": nnz.value,
+            "bytes_model_lower_bound": int(nbytes), "gbs_lower_bound": round(nbytes / secs.value / 1e9, 1),
             "tiles": nt.value, "preconditioner": "jacobi (right)", "build_s": round(t_build, 3),
             "solve_s": round(secs.value, 3), "cycles": cyc.value, "converged": int(cv.sum()),
             "iters_min_mean_max": [int(it.min()), round(float(it.mean()), 1), int(it.max())],

@@ -33,22 +33,22 @@ KINDS = {"reduce": 0, "map": 1, "reduce1": 2}
 DTYPES = {"f64": np.float64, "f32": np.float32}
 
 
+MAX_GRID = 4096   # hpb_set_tuning's limit on grid_max
+
+
 def candidates(kind, num_cu):
+    """Launch shapes (block, unroll, grid_max, flags) tried for a kernel family."""
+    mults = (0.5, 0.75, 1, 1.5, 2, 4) if kind in ("reduce", "reduce1") else (1, 2, 4, 8, 16)
     out = []
-    if kind in ("reduce", "reduce1"):
-        for block in (256, 512):
-            for unroll in (2, 4, 8):
-                for mult in (0.5, 0.75, 1, 1.5, 2, 4):
-                    for flags in (0, 1):
-                        out.append((block, unroll, max(1, int(num_cu * mult)), flags))
-    else:
-        for block in (256, 512):
-            for unroll in (1, 2, 4):
-                if block == 512 and unroll == 1:
-                    continue
-                for mult in (1, 2, 4, 8, 16):
-                    for flags in (0, 1):
-                        out.append((block, unroll, int(num_cu * mult), flags))
+    for block in (256, 512):
+        for unroll in ((2, 4, 8) if kind in ("reduce", "reduce1") else (1, 2, 4)):
+            if kind == "map" and block == 512 and unroll == 1:
+                continue
+            for mult in mults:
+                for flags in (0, 1):
+                    cand = (block, unroll, min(MAX_GRID, max(1, int(num_cu * mult))), flags)
+                    if cand not in out:
+                        out.append(cand)
     return out
 
 

@@ -49,6 +49,9 @@ struct dstate {
     int done, stop, converged, breakdown, invalid;
     int cycle_len, hist_len, hist_cap, recalc_freq, skip_recalc;
     int early_exit, basis;
+    // reliable updates: the FP32 residual is the true one / rscale, and x64 gains
+    // xscale * x32 (both powers of two: 1 in every other solve)
+    double rscale, xscale;
     double wn2;   // GMRES selective reorthogonalisation: ||w||^2 before the projection
     int skip1;    // ... 1 when the second Gram-Schmidt pass is not needed (or the cycle stopped)
 };
@@ -123,6 +126,23 @@ template <typename T>
 __global__ __launch_bounds__(DB) void k_fill(size_t n, T* __restrict__ v, T a)
 {
     for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) v[i] = a;
+}
+
+// max |x_i|: non-negative doubles order like their bit patterns, so blocks fold
+// their maxima with one integer atomic max (order-independent)
+__global__ __launch_bounds__(DB) void k_absmax(size_t n, const double* __restrict__ x,
+                                               unsigned long long* __restrict__ out)
+{
+    double m = 0.0;
+    for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) m = fmax(m, fabs(x[i]));
+    for (int k = 32; k >= 1; k >>= 1) m = fmax(m, __shfl_xor(m, k, 64));
+    __shared__ double wm[DB / 64];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < DB / 64; ++w) m = fmax(m, wm[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // ---------------------------------------------------------------- GMRES kernels
@@ -684,7 +704,7 @@ __device__ void epi_cycle(int kind, dstate* st, double d0, double d1, T* coef, d
 {
     st->iters += st->basis;
     st->cycles++;
-    const double rn = sqrt(d1);
+    const double rn = sqrt(d1) * st->rscale;
     if (kind == EPI_CHECK) {
         monitor_check(st, hist, rn);
         if (st->done) {
@@ -924,7 +944,7 @@ __global__ void k_rel_check(dstate* st, const T* n4, double delta)
     if (st->stop) return;
     st->iters += st->basis;
     st->cycles++;
-    const double rn = sqrt(dval(n4));
+    const double rn = sqrt(dval(n4)) * st->rscale;
     st->rnorm = rn;
     if (!finite(rn)) {
         st->invalid = 1;
@@ -935,13 +955,14 @@ __global__ void k_rel_check(dstate* st, const T* n4, double delta)
     if (rn <= delta * st->rlast || rn <= st->tol || st->iters >= st->max_iters) st->stop = 1;
 }
 
-// x64 += x32, x32 = 0
+// x64 += xscale x32, x32 = 0
 __global__ __launch_bounds__(DB) void k_rel_flush(size_t n, float* __restrict__ x32, double* __restrict__ x64,
-                                                  const int* __restrict__ skip)
+                                                  const dstate* __restrict__ st, const int* __restrict__ skip)
 {
     if (*skip) return;
+    const double g = st->xscale;
     for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
-        x64[i] += (double)x32[i];
+        x64[i] += g * (double)x32[i];
         x32[i] = 0.0f;
     }
 }
@@ -950,10 +971,17 @@ __global__ __launch_bounds__(DB) void k_rel_flush(size_t n, float* __restrict__ 
 // When the update did not lower the true residual the FP32 recurrences have lost
 // their accuracy: the iteration restarts from the new residual (r~, u_0, rho,
 // alpha, omega as at the start).
-__global__ void k_rel_after(dstate* st, const double* n4, double* hist, int first)
+// On a (re)start the FP32 residual scale becomes the power of two at or below ||r||:
+// the FP32 iteration then runs on the same significands with exponents near 0,
+// and with the matrix scaled to |a_ij| < 1 (mscale) A^L r cannot overflow.
+__global__ void k_rel_after(dstate* st, const double* n4, double* hist, int first, double mscale)
 {
     if (st->done) return;
     const double rn = sqrt(dval(n4));
+    if (first && rn > 0.0 && finite(rn)) {
+        st->rscale = ldexp(1.0, ilogb(rn));
+        st->xscale = st->rscale * mscale;
+    }
     st->restart = first || !(rn < st->rlast);
     st->rlast = rn;
     monitor_check(st, hist, rn);
@@ -965,6 +993,16 @@ __global__ void k_rel_after(dstate* st, const double* n4, double* hist, int firs
     st->stop = st->done;
 }
 
+// a breakdown of the FP32 iteration (rho = 0, or a Gram matrix that is not
+// numerically positive definite) in a reliable-update solve: clear the stop flags
+// so that an FP64 update and a restart from the true residual follow
+__global__ void k_rel_resume(dstate* st)
+{
+    st->breakdown = 0;
+    st->done = 0;
+    st->stop = 0;
+}
+
 // r32 = (float) r64; on a restart also r~ = r32 / ||r32|| and u_0 = 0
 __global__ __launch_bounds__(DB) void k_rel_convert(size_t n, const double* __restrict__ r64, float* __restrict__ r32,
                                                     float* __restrict__ rt, float* __restrict__ u0,
@@ -972,9 +1010,10 @@ __global__ __launch_bounds__(DB) void k_rel_convert(size_t n, const double* __re
 {
     if (st->done) return;
     const double inv = st->rlast > 0.0 ? 1.0 / st->rlast : 1.0;
+    const double rs = 1.0 / st->rscale;   // a power of two: exact
     const bool restart = st->restart != 0;
     for (size_t i = (size_t)blockIdx.x * DB + threadIdx.x; i < n; i += (size_t)gridDim.x * DB) {
-        const float v = (float)r64[i];
+        const float v = (float)(r64[i] * rs);
         r32[i] = v;
         if (restart) {
             rt[i] = (float)(r64[i] * inv);
@@ -1100,6 +1139,8 @@ template <typename T>
 int init_state(hpb_handle_t h, dstate* d, const hpb_solver_params& p, int basis, int hist_cap)
 {
     dstate s{};
+    s.rscale = 1.0;
+    s.xscale = 1.0;
     s.max_iters = p.max_iters;
     s.min_iters = p.min_iters;
     s.recalc_freq = p.resid_recalc_freq;
@@ -1124,6 +1165,7 @@ struct rel64 {
     const double* b;
     double* x;
     double delta;
+    double mscale;   // the FP32 values are mscale * val (a power of two)
 };
 
 template <typename T>
@@ -1548,7 +1590,7 @@ private:
 
     // FP64 update: x64 += x32, x32 = 0, r64 = b64 - A64 x64 (compensated rows),
     // monitor check on ||r64||, r32 = (float) r64 (and on the first call r~, u_0)
-    int rel_update(int first)
+    int rel_update(int first, int force_restart = 0)
     {
         hpb_handle_t h = P.h;
         hipStream_t st = h->stream;
@@ -1556,10 +1598,10 @@ private:
         const unsigned vg = vgrid(h, n);
         dstate* S = dst.p;
         if (!first)
-            hipLaunchKernelGGL(k_rel_flush, dim3(vg), dim3(DB), 0, st, n, (float*)P.x, R.x, &S->done);
+            hipLaunchKernelGGL(k_rel_flush, dim3(vg), dim3(DB), 0, st, n, (float*)P.x, R.x, S, &S->done);
         DS_CHECK(csr_apply<double>(h, P.rows, P.nnz, P.rp, P.ci, R.val, R.x, R.b, r64.p, true, &S->done));
         DS_CHECK(nrm2_async<double>(h, n, r64.p, n4d.p, true, &S->done));
-        hipLaunchKernelGGL(k_rel_after, dim3(1), dim3(1), 0, st, S, n4d.p, hist(), first);
+        hipLaunchKernelGGL(k_rel_after, dim3(1), dim3(1), 0, st, S, n4d.p, hist(), first | force_restart, R.mscale);
         hipLaunchKernelGGL(k_rel_convert, dim3(vg), dim3(DB), 0, st, n, r64.p, (float*)V[0], (float*)rt.p,
                            (float*)Ub[0].p, S);
         HPB_LAUNCH_CHECK();
@@ -1580,20 +1622,32 @@ private:
         DS_CHECK(read_state<T>(h, S, out));
         const int max_cycles = prm.max_iters / L + 2;
         const int every = 4;
-        int cycles = 0;
-        while (!out->done && cycles < max_cycles) {
+        constexpr int max_breakdown_restarts = 16;
+        int cycles = 0, restarts = 0;
+        while (cycles < max_cycles) {
+            if (out->done) {
+                // a breakdown restarts from an FP64 update; anything else ends the solve
+                if (!out->breakdown || out->converged || out->invalid || out->iters >= prm.max_iters ||
+                    restarts >= max_breakdown_restarts)
+                    break;
+                hipLaunchKernelGGL(k_rel_resume, dim3(1), dim3(1), 0, st, S);
+                DS_CHECK(rel_update(0, 1));
+                DS_CHECK(read_state<T>(h, S, out));
+                restarts++;
+                updates++;
+                continue;
+            }
             for (int k = 0; k < every; ++k) HPB_HIP(hipGraphLaunch(gg.e, st));
             cycles += every;
             DS_CHECK(read_state<T>(h, S, out));
-            if (out->done) break;
-            if (out->stop) {
+            if (!out->done && out->stop) {
                 DS_CHECK(rel_update(0));
                 DS_CHECK(read_state<T>(h, S, out));
                 updates++;
             }
         }
         // keep the progress of an unfinished run (x32 holds the increment since the last update)
-        hipLaunchKernelGGL(k_rel_flush, dim3(vgrid(h, n)), dim3(DB), 0, st, n, (float*)P.x, R.x, &S->invalid);
+        hipLaunchKernelGGL(k_rel_flush, dim3(vgrid(h, n)), dim3(DB), 0, st, n, (float*)P.x, R.x, S, &S->invalid);
         HPB_LAUNCH_CHECK();
         return HPB_OK;
     }
@@ -1674,7 +1728,7 @@ void dsolve_release(dsolver_base* cache) { delete cache; }
 // rel_tol is the reduction delta of the FP32 recursive residual that triggers an
 // FP64 update; rel_tol is the final tolerance on ||b - A x|| / ||b||.
 int dsolve_reliable(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, const float* val32,
-                    const double* val64, const double* b64, double* x64, float* r32, float* x32,
+                    double mscale, const double* val64, const double* b64, double* x64, float* r32, float* x32,
                     const hpb_solver_params& inner, double rel_tol, hpb_solver_result* res, int* updates)
 {
     HPB_REQUIRE(inner.solver == HPB_SOLVER_BICGSTABL && inner.basis >= 1 && inner.basis <= MAXL, HPB_ERR_UNSUPPORTED,
@@ -1685,7 +1739,7 @@ int dsolve_reliable(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const 
     hpb_solver_params p = inner;
     const double delta = p.rel_tol;
     p.rel_tol = rel_tol;   // the monitor tests the FP64 true residual against the final tolerance
-    rel64 R{val64, b64, x64, delta};
+    rel64 R{val64, b64, x64, delta, mscale};
     dsolver<float> ds;
     DS_CHECK(ds.setup(P, p, 0, &R));
     dstate out{};
@@ -1697,6 +1751,25 @@ int dsolve_reliable(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const 
     res->rhs_norm = out.bnorm;
     res->history_len = 0;
     if (updates) *updates = ds.updates;
+    return HPB_OK;
+}
+
+// 2^-e with max |x_i| 2^-e in [0.5, 1): scaling FP64 values by it before the
+// conversion to FP32 changes no significand, and keeps |a_ij| < 1 in the FP32 copy
+int absmax_pow2_scale(hpb_handle_t h, size_t n, const double* x, double* scale)
+{
+    HPB_REQUIRE(h && scale && (n == 0 || x), HPB_ERR_INVALID, "absmax: null argument");
+    *scale = 1.0;
+    if (n == 0) return HPB_OK;
+    auto* d = reinterpret_cast<unsigned long long*>(h->dres);
+    HPB_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), h->stream));
+    hipLaunchKernelGGL(k_absmax, dim3(vgrid(h, n)), dim3(DB), 0, h->stream, n, x, d);
+    HPB_LAUNCH_CHECK();
+    HPB_HIP(hipMemcpyAsync(h->hres, d, sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+    HPB_HIP(hipStreamSynchronize(h->stream));
+    double m;
+    std::memcpy(&m, h->hres, sizeof m);
+    if (m > 0.0 && std::isfinite(m)) *scale = std::ldexp(1.0, -(std::ilogb(m) + 1));
     return HPB_OK;
 }
 

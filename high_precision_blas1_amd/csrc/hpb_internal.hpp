@@ -70,8 +70,10 @@ int dsolve(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, 
 void dsolve_release(dsolver_base* cache);
 // FP32 BiCGStab(L) iterations with reliable updates of an FP64 solution
 int dsolve_reliable(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* ci, const float* val32,
-                    const double* val64, const double* b64, double* x64, float* r32, float* x32,
+                    double mscale, const double* val64, const double* b64, double* x64, float* r32, float* x32,
                     const hpb_solver_params& inner, double rel_tol, hpb_solver_result* res, int* updates);
+// power-of-two scale s with max |x_i| s in [0.5, 1) (1 for an empty or zero x)
+int absmax_pow2_scale(hpb_handle_t h, size_t n, const double* x, double* scale);
 // b == null: y = a A x with a = *dev_alpha (1 when dev_alpha is null); else y = b - A x
 template <typename T>
 int csr_apply(hpb_handle_t h, int rows, int64_t nnz, const int* rp, const int* col, const T* val, const T* x,

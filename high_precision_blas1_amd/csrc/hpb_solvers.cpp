@@ -249,7 +249,14 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
             void* p;
             ~dev_guard() { hpb_free(p); }
         } val32_guard{val32};
-        hpb::check(hpb_convert_f64_to_f32(h, (size_t)nnz, val, val32), "ir: convert matrix");
+        // FP32 values scaled by a power of two to |a_ij| < 1 (no significand changes):
+        // BiCGStab(L)'s A^L r stays far from FP32 overflow on badly scaled operators
+        double mscale = 1.0;
+        {
+            const int rc = hpb::absmax_pow2_scale(h, (size_t)nnz, val, &mscale);
+            if (rc != HPB_OK) return rc;
+        }
+        hpb::check(hpb_scale_convert_f64_to_f32(h, (size_t)nnz, mscale, val, val32), "ir: convert matrix");
         // FP32 values; the index arrays stay the caller's (not owned)
         s32::mat_t A32(h);
         A32.adopt_device(rows, rows, nnz, (int*)row_ptr, (int*)col_ind, val32, false);
@@ -288,8 +295,8 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
             // reliable updates: one FP32 BiCGStab(L) iteration feeding an FP64 solution
             hpb_solver_result ir{};
             int updates = 0;
-            const int rc = hpb::dsolve_reliable(h, rows, nnz, row_ptr, col_ind, val32, val, b, x, r32, d32, *inner,
-                                                rel_tol, &ir, &updates);
+            const int rc = hpb::dsolve_reliable(h, rows, nnz, row_ptr, col_ind, val32, mscale, val, b, x, r32, d32,
+                                                *inner, rel_tol, &ir, &updates);
             if (rc != HPB_OK) return rc;
             HPB_HIP(hipStreamSynchronize(h->stream));
             res->seconds = now_s() - t0;
@@ -314,7 +321,7 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
             }
             if (outer >= max_outer) break;
             ++outer;
-            // solve A d = r / ||r|| in FP32 to the inner tolerance
+            // solve (mscale A) d = r / ||r|| in FP32 to the inner tolerance
             hpb::check(hpb_scale_convert_f64_to_f32(h, n, 1.0 / rn, r, r32), "ir: scale residual");
             vo32.assign_scalar(0.0f, d32);
             hpb_solver_result ir{};
@@ -333,7 +340,8 @@ int hpb_csr_solve_ir_f64(hpb_handle_t h, int rows, int64_t nnz, const int* row_p
                 run_solver<float>(*inner, &log, vo32, op32, r32, d32, &ir, nullptr, 0, inner->rel_tol);
             }
             inner_total += ir.iters;
-            hpb::check(hpb_axpy_f32_to_f64(h, n, rn, d32, x), "ir: update");
+            // (mscale A) d = r / ||r||  =>  A (||r|| mscale d) = r
+            hpb::check(hpb_axpy_f32_to_f64(h, n, rn * mscale, d32, x), "ir: update");
         }
         HPB_HIP(hipStreamSynchronize(h->stream));
         res->seconds = now_s() - t0;

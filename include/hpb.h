@@ -368,6 +368,8 @@ typedef struct hpb_solver_params {
                                 1 (default) reliable updates -- one FP32 iteration that is never
                                 restarted; each time its recursive residual has dropped by the inner
                                 rel_tol, x64 += x32 and r = b - A x is recomputed in FP64 and fed back;
+                                a breakdown of the FP32 iteration also triggers that update, followed by a
+                                restart from the true residual (at most 16 times);
                                 0 restarted refinement (a fresh inner solve per sweep) */
 } hpb_solver_params;
 typedef struct hpb_solver_result {

@@ -182,3 +182,25 @@ def test_reliable_update_refinement(handle, L, delta):
     res0 = S.solve_ir(handle, A, db, dx, inner0, rel_tol=1e-12, max_outer=30)
     assert res0["converged"] == 1 and res0["true_rel_resid"] <= 1e-12
     A.free()
+
+
+@pytest.mark.parametrize("L", [2, 4, 8])
+@pytest.mark.parametrize("refinement", [1, 0])
+def test_mixed_precision_badly_scaled_operator(handle, L, refinement):
+    """Anisotropic diffusion with k_z / k_x = 1e4 (diagonal 2e4 + 4): unscaled, the
+    FP32 BiCGStab(L) vectors A^j r grow like (4e4)^j and their Gram entries overflow
+    FP32 at L = 4 (the reliable-update solve then broke down in its first cycle).
+    The FP32 copy of the matrix and the FP32 residual are scaled by powers of two,
+    which changes no significand: every variant reaches 1e-12."""
+    Mh, A, b = _problem(handle, 32, 32, 32, S.ANISO7)
+    b = b * 1e3                     # a right-hand side of norm ~1e8
+    say("scaled mixed", L, refinement)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    inner = S.params(solver=S.BICGSTABL, basis=L, rel_tol=0.1 if refinement else 1e-4, max_iters=20000,
+                     refinement=refinement)
+    res = S.solve_ir(handle, A, db, dx, inner, rel_tol=1e-12, max_outer=200)
+    assert res["converged"] == 1 and res["true_rel_resid"] <= 1e-12, res
+    assert res["iters"] > 0
+    assert _host_rel(Mh, b, dx.numpy()) <= 1.1e-12
+    A.free()
