@@ -411,7 +411,7 @@ def _mixed_on(H, h, S, log, nx, coef, seed, plain, mixed):
     return res
 
 
-def cfg_mixed(H, h, log):
+def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
     """FP32-storage inner GMRES / BiCGStab(2) with compensated FP32 dots inside FP64
     refinement vs the plain FP64 compensated solvers, time to 1e-12 true residual.
     GMRES: restarted refinement, one FP32 GMRES solve per sweep to the inner
@@ -422,22 +422,24 @@ def cfg_mixed(H, h, log):
     res = {}
     G = dict(solver=S.GMRES, high_precision=1)
     B = dict(solver=S.BICGSTABL, basis=2, high_precision=1)
-    log("mixed precision 128^3 convection-diffusion")
-    res["convdiff128"] = _mixed_on(
-        H, h, S, log, 128, S.CONVDIFF7, 2,
-        {"gmres30_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=30, **G),
-         "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
-        {"gmres30_fp32_refined": S.params(rel_tol=1e-4, max_iters=5000, restart=30, **G),
-         "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=20000, refinement=1, **B)})
-    res["convdiff128"]["matrix"] = "128^3 convection-diffusion (as krylov)"
-    log("mixed precision 256^3 anisotropic diffusion")
-    res["aniso256"] = _mixed_on(
-        H, h, S, log, 256, S.ANISO7, 2,
-        {"gmres50_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=50, **G),
-         "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
-        {"gmres50_fp32_refined": S.params(rel_tol=1e-2, max_iters=5000, restart=50, compensated_spmv=1, **G),
-         "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, **B)})
-    res["aniso256"]["matrix"] = "256^3 7-point anisotropic diffusion, k_z / k_x = 1e4, Dirichlet"
+    if "convdiff128" in problems:
+        log("mixed precision 128^3 convection-diffusion")
+        res["convdiff128"] = _mixed_on(
+            H, h, S, log, 128, S.CONVDIFF7, 2,
+            {"gmres30_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=30, **G),
+             "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
+            {"gmres30_fp32_refined": S.params(rel_tol=1e-4, max_iters=5000, restart=30, **G),
+             "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=20000, refinement=1, **B)})
+        res["convdiff128"]["matrix"] = "128^3 convection-diffusion (as krylov)"
+    if "aniso256" in problems:
+        log("mixed precision 256^3 anisotropic diffusion")
+        res["aniso256"] = _mixed_on(
+            H, h, S, log, 256, S.ANISO7, 2,
+            {"gmres50_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=50, **G),
+             "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
+            {"gmres50_fp32_refined": S.params(rel_tol=1e-2, max_iters=5000, restart=50, compensated_spmv=1, **G),
+             "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, **B)})
+        res["aniso256"]["matrix"] = "256^3 7-point anisotropic diffusion, k_z / k_x = 1e4, Dirichlet"
     return res
 
 
@@ -511,18 +513,33 @@ def cfg_batch(H, h, log, nsys=512):
             "max_true_rel_resid": float(rr.max())}
 
 
-def run_suite(H, h, x, y, n, args, log, no_cpu=False):
+SUITES = ("cpu_plain", "blas1", "cond_sweep", "spmv", "krylov", "mixed", "mixed_convdiff128", "mixed_aniso256",
+          "multidot", "batch")
+
+
+def run_suite(H, h, x, y, n, args, log, no_cpu=False, only=None):
+    """The configurations after the headline; `only` restricts them to a subset
+    (a profiling aid: one solver workload under rocprofv3)."""
+    want = set(only) if only else set(SUITES)
     res = {}
-    if not no_cpu:
+    if "cpu_plain" in want and not no_cpu:
         log("cpu plain config")
         res["cpu_plain"] = cfg_cpu_plain()
-    res["blas1"] = cfg_blas1(H, h, x, y, n)
-    res["cond_sweep"] = cfg_cond_sweep(H, h, n, log)
-    res["spmv"] = cfg_spmv(H, h, log)
-    res["krylov"] = cfg_krylov(H, h, log)
-    res["mixed"] = cfg_mixed(H, h, log)
-    res["multidot"] = cfg_multidot(H, h, log)
-    res["batch"] = cfg_batch(H, h, log)
+    if "blas1" in want:
+        res["blas1"] = cfg_blas1(H, h, x, y, n)
+    if "cond_sweep" in want:
+        res["cond_sweep"] = cfg_cond_sweep(H, h, n, log)
+    if "spmv" in want:
+        res["spmv"] = cfg_spmv(H, h, log)
+    if "krylov" in want:
+        res["krylov"] = cfg_krylov(H, h, log)
+    probs = [p for p in ("convdiff128", "aniso256") if "mixed" in want or "mixed_" + p in want]
+    if probs:
+        res["mixed"] = cfg_mixed(H, h, log, probs)
+    if "multidot" in want:
+        res["multidot"] = cfg_multidot(H, h, log)
+    if "batch" in want:
+        res["batch"] = cfg_batch(H, h, log)
     return res
 
 
@@ -538,7 +555,16 @@ def main():
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--op", default="dot2", choices=("dot2", "copy", "nrm2c", "exact"),
                     help="kernel timed for the headline (copy: the plain copy used to calibrate HBM counters)")
+    ap.add_argument("--suite", default=None,
+                    help="comma-separated subset of the configurations after the headline: " + ",".join(SUITES))
     args = ap.parse_args()
+    only = None
+    if args.suite:
+        only = [v for v in args.suite.split(",") if v]
+        bad = [v for v in only if v not in SUITES]
+        if bad:
+            print("bench.py: unknown --suite entries %s (choose from %s)" % (bad, ",".join(SUITES)), file=sys.stderr)
+            sys.exit(2)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the hot path is measured on one MI355X; "
               "--gpus %d is not supported)" % args.gpus, file=sys.stderr)
@@ -604,7 +630,7 @@ def main():
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(H, h, x, y, n)
     if not args.headline_only:
-        out["configs"] = run_suite(H, h, x, y, n, args, log, no_cpu=args.no_cpu)
+        out["configs"] = run_suite(H, h, x, y, n, args, log, no_cpu=args.no_cpu, only=only)
     x.free()
     y.free()
     dout.free()
