@@ -139,6 +139,15 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
         core = ts[1:-1]  # drop min and max (test_vector_operations.cpp:44-49)
         return sum(core) / len(core), len(ts), core[0], core[-1]
 
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     t_thr, reps_thr, tmin, tmax = run(fthr, budget_s * 0.6)
     t_seq, reps_seq, _, _ = run(fseq, budget_s * 0.4)
     nbytes = 16.0 * sample
@@ -146,6 +155,7 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
         "value": round(nbytes / t_thr / 1e9, 3),
         "unit": "GB/s",
         "cores": cores,
+        "cpu_model": model,
         "kind": kind,
         "sample": "FP64 Dot2 over the first %d elements of the benchmark vectors (%d MiB), threaded_reduction "
                   "with %d threads, mean of %d runs without min/max" % (sample, int(nbytes) >> 20, cores, reps_thr),
@@ -340,7 +350,19 @@ def _solve_bytes(p, n, nnz, iters, sval=8, s=8, recalc=True):
     return _gmres_bytes(p.restart, n, nnz, sval, s, iters)
 
 
-def _run_solve(H, h, S, A, b, xs, p, ir=False):
+def _rel_diff(H, h, n, a, b):
+    """||a - b|| / ||b|| with compensated FP64 norms (device vectors)."""
+    import numpy as np
+    e = H.DeviceArray(n, np.float64, h)
+    H.call("hpb_axpbyc_f64", h.ptr, n, ctypes.c_double(1.0), a.ptr, ctypes.c_double(-1.0), b.ptr, e.ptr)
+    o1, o2 = np.zeros(4), np.zeros(4)
+    H.call("hpb_nrm2c_f64", h.ptr, n, e.ptr, o1.ctypes.data_as(ctypes.c_void_p))
+    H.call("hpb_nrm2c_f64", h.ptr, n, b.ptr, o2.ctypes.data_as(ctypes.c_void_p))
+    e.free()
+    return float(o1[3] / o2[3])
+
+
+def _run_solve(H, h, S, A, b, xs, p, ir=False, keep_x=False):
     import numpy as np
     x = H.DeviceArray(A.rows, np.float64, h)
     H.call("hpb_fill_f64", h.ptr, A.rows, ctypes.c_double(0.0), x.ptr)
@@ -351,10 +373,15 @@ def _run_solve(H, h, S, A, b, xs, p, ir=False):
     o1, o2 = np.zeros(4), np.zeros(4)
     H.call("hpb_nrm2c_f64", h.ptr, A.rows, e.ptr, o1.ctypes.data_as(ctypes.c_void_p))
     H.call("hpb_nrm2c_f64", h.ptr, A.rows, xs.ptr, o2.ctypes.data_as(ctypes.c_void_p))
-    x.free()
+    if keep_x:
+        kept = x
+    else:
+        x.free()
     e.free()
     out = {k: (round(v, 6) if isinstance(v, float) and k == "seconds" else v)
            for k, v in r.items() if k in ("converged", "iters", "outer_iters", "true_rel_resid", "seconds")}
+    if keep_x:
+        out["_x"] = kept
     out["rel_err_vs_xstar"] = float(o1[3] / o2[3])
     n, nnz = A.rows, A.nnz
     if ir:
@@ -398,14 +425,18 @@ def _mixed_on(H, h, S, log, nx, coef, seed, plain, mixed):
     res = {"rows": A.rows, "nnz": A.nnz}
     for k, p in plain.items():
         log(k)
-        res[k] = _run_solve(H, h, S, A, b, xs, p)
+        res[k] = _run_solve(H, h, S, A, b, xs, p, keep_x=True)
     for k, p in mixed.items():
         log(k)
-        res[k] = _run_solve(H, h, S, A, b, xs, p, ir=True)
+        res[k] = _run_solve(H, h, S, A, b, xs, p, ir=True, keep_x=True)
     for a, m in zip(plain, mixed):
+        # the mixed-precision solution against the FP64 solve of the same solver
+        res[m]["rel_diff_vs_fp64_solve"] = _rel_diff(H, h, A.rows, res[m]["_x"], res[a]["_x"])
         if res[a]["converged"] and res[m]["converged"]:
             res["speedup_" + m] = round(res[a]["seconds"] / res[m]["seconds"], 3)
             res["bytes_ratio_" + m] = round(res[a]["bytes_moved"] / res[m]["bytes_moved"], 3)
+    for k in list(plain) + list(mixed):
+        res[k].pop("_x").free()
     for a in (A, b, xs):
         a.free()
     return res
