@@ -544,8 +544,111 @@ def cfg_batch(H, h, log, nsys=512):
             "max_true_rel_resid": float(rr.max())}
 
 
+def cfg_next(H, h, log):
+    """The components after the hot path, measured on the 128^3 convection-diffusion
+    matrix (2 097 152 rows, 14 581 760 non-zeros) unless stated:
+    F1 MatrixMarket write + read of the 64^3 matrix of the same operator (262 144 rows,
+       host, a temporary file; the round trip must reproduce every array bit for bit);
+    F3 level-scheduled triangular solves (lower with unit diagonal, upper with the stored
+       diagonal), the ILU(0) factorisation, and ILU(0)-preconditioned BiCGStab(2) and
+       GMRES(30) to 1e-12 against the unpreconditioned device solvers;
+    F4 plain BiCGStab to 1e-12, and the complex Dot2 x^H y at n = 2^26 complex numbers
+       (2 GiB read per call).
+    F2, the exact dot, is timed in the condition sweep."""
+    import tempfile
+    import numpy as np
+    import high_precision_blas1_amd.solvers as S
+    L = H.lib()
+    out = {}
+    # F1
+    log("next: MatrixMarket round trip")
+    A = S.DeviceCSR.stencil7(h, 64, 64, 64, S.CONVDIFF7)
+    n, nnz = A.rows, A.nnz
+    rp, ci, va = A.rp.numpy(), A.col.numpy(), A.val.numpy()
+    A.free()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "convdiff128.mtx").encode()
+        t0 = time.perf_counter()
+        H.call("hpb_mm_write_csr", path, n, n, nnz, rp.ctypes.data_as(ctypes.c_void_p),
+               ci.ctypes.data_as(ctypes.c_void_p), va.ctypes.data_as(ctypes.c_void_p))
+        t_w = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        rows_, cols_, nnz_ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        prp, pci, pva = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        t0 = time.perf_counter()
+        H.call("hpb_mm_read_csr", path, ctypes.byref(rows_), ctypes.byref(cols_), ctypes.byref(nnz_),
+               ctypes.byref(prp), ctypes.byref(pci), ctypes.byref(pva))
+        t_r = time.perf_counter() - t0
+    same = (rows_.value == n and nnz_.value == nnz and
+            np.ctypeslib.as_array(ctypes.cast(prp, ctypes.POINTER(ctypes.c_int)), (n + 1,)).tobytes() == rp.tobytes()
+            and np.ctypeslib.as_array(ctypes.cast(pci, ctypes.POINTER(ctypes.c_int)), (nnz,)).tobytes() == ci.tobytes()
+            and np.ctypeslib.as_array(ctypes.cast(pva, ctypes.POINTER(ctypes.c_double)), (nnz,)).tobytes()
+            == va.tobytes())
+    for q in (prp, pci, pva):
+        L.hpb_mm_free(q)
+    out["matrix_market"] = {"rows": n, "nnz": nnz, "file_mb": round(size / 1e6, 1), "write_s": round(t_w, 3),
+                            "read_s": round(t_r, 3), "read_mb_per_s": round(size / t_r / 1e6, 1),
+                            "round_trip_bit_equal": bool(same)}
+    A, b, xs = _solve_problem(H, h, S, 128, S.CONVDIFF7, 2)
+    n, nnz = A.rows, A.nnz
+    # F3
+    log("next: triangular solves and ILU(0)")
+    tri = {}
+    x1 = H.DeviceArray(n, np.float64, h)
+    for uplo, name in ((0, "lower_unit"), (1, "upper")):
+        t = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        H.call("hpb_csr_trsv_analysis", h.ptr, uplo, n, nnz, A.rp.ptr, A.col.ptr, ctypes.byref(t))
+        t_an = time.perf_counter() - t0
+        nl = ctypes.c_int()
+        H.call("hpb_csr_trsv_levels", t, ctypes.byref(nl))
+        for comp in (0, 1):
+            ms = timed(h, L.hpb_csr_trsv_f64, (h.ptr, t, A.val.ptr, ctypes.c_double(1.0), b.ptr, x1.ptr, comp), 10, 2)
+            tri["%s%s_ms" % (name, "_compensated" if comp else "")] = round(ms, 4)
+        tri[name + "_levels"] = nl.value
+        tri[name + "_analysis_s"] = round(t_an, 3)
+        if uplo == 0:
+            lu = H.DeviceArray(nnz, np.float64, h)
+            H.call("hpb_memcpy_d2d", h.ptr, lu.ptr, A.val.ptr, nnz * 8)
+            h.sync()
+            h.timer_start()
+            H.call("hpb_csr_ilu0_f64", h.ptr, t, lu.ptr)
+            tri["ilu0_factor_ms"] = round(h.timer_stop(), 3)
+            lu.free()
+        H.call("hpb_trsv_destroy", t)
+    x1.free()
+    out["triangular"] = tri
+    sol = {}
+    G = dict(solver=S.GMRES, restart=30, high_precision=1)
+    B = dict(solver=S.BICGSTABL, basis=2, high_precision=1)
+    for k, p in (("bicgstab2", S.params(rel_tol=1e-12, max_iters=20000, **B)),
+                 ("bicgstab2_ilu0", S.params(rel_tol=1e-12, max_iters=20000, preconditioner=S.PRECOND_ILU0, **B)),
+                 ("gmres30", S.params(rel_tol=1e-12, max_iters=20000, **G)),
+                 ("gmres30_ilu0", S.params(rel_tol=1e-12, max_iters=20000, preconditioner=S.PRECOND_ILU0, **G)),
+                 ("bicgstab_plain", S.params(rel_tol=1e-12, max_iters=20000, solver=S.BICGSTAB, high_precision=1))):
+        log("next: " + k)
+        r = _run_solve(H, h, S, A, b, xs, p)
+        sol[k] = {q: r[q] for q in ("converged", "iters", "seconds", "true_rel_resid", "rel_err_vs_xstar")}
+    out["solvers"] = sol
+    for a in (A, b, xs):
+        a.free()
+    # F4 complex dot
+    log("next: complex Dot2")
+    nc = 1 << 26
+    zx = H.DeviceArray(2 * nc, np.float64, h)
+    zy = H.DeviceArray(2 * nc, np.float64, h)
+    H.call("hpb_random_uniform_f64", h.ptr, 2 * nc, 11, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), zx.ptr)
+    H.call("hpb_random_uniform_f64", h.ptr, 2 * nc, 11, 1, ctypes.c_double(-1.0), ctypes.c_double(1.0), zy.ptr)
+    d6 = H.DeviceArray(8, np.float64, h)
+    ms = timed(h, L.hpb_zdot2_f64_async, (h.ptr, nc, zx.ptr, zy.ptr, d6.ptr), 20, 3)
+    out["complex_dot2"] = {"n_complex": nc, "ms": round(ms, 4), "gbs_algorithmic": round(32.0 * nc / ms / 1e6, 1)}
+    for a in (zx, zy, d6):
+        a.free()
+    return out
+
+
 SUITES = ("cpu_plain", "blas1", "cond_sweep", "spmv", "krylov", "mixed", "mixed_convdiff128", "mixed_aniso256",
-          "multidot", "batch")
+          "multidot", "batch", "next")
 
 
 def run_suite(H, h, x, y, n, args, log, no_cpu=False, only=None):
@@ -571,6 +674,8 @@ def run_suite(H, h, x, y, n, args, log, no_cpu=False, only=None):
         res["multidot"] = cfg_multidot(H, h, log)
     if "batch" in want:
         res["batch"] = cfg_batch(H, h, log)
+    if "next" in want:
+        res["next"] = cfg_next(H, h, log)
     return res
 
 
