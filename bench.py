@@ -714,7 +714,9 @@ def main():
     log("generating GenDot pair n=%d kappa=%g" % (n, args.cond))
     x = H.DeviceArray(n, np.float64, h)
     y = H.DeviceArray(n, np.float64, h)
+    t_gen = time.perf_counter()
     exact, dd = gendot(H, h, n, args.cond, args.seed, x, y)
+    t_gen = time.perf_counter() - t_gen
     dout = H.DeviceArray(4, np.float64, h)
     if args.op != "dot2":
         # profiling aid: time another kernel over the same vectors and stop
@@ -760,7 +762,7 @@ def main():
                      "kernel": "hpb::k_reduce<double, DOT2> (hpb_dot2_f64)",
                      "algorithmic_bytes_per_launch": int(nbytes)},
         "result": {"dot2": float(res[2]), "exact": float(exact), "rel_err": rel_err(res[2], exact),
-                   "cond_achieved": float(dd[2])},
+                   "cond_achieved": float(dd[2]), "generator_s": round(t_gen, 3)},
     }
     if not args.no_cpu:
         log("CPU baseline")

@@ -12,7 +12,7 @@
 //   s = r - alpha nu, t = P A s, omega = (t, s) / (t, t),
 //   x += alpha p + omega s, r = s - omega t (or b - A x every recalc_freq steps).
 // Difference: a NaN or infinite coefficient stops the iteration and is reported
-// by breakdown(); solve() then returns the monitor's verdict (false unless the
+// by breakdown(); solve() then returns the monitor's decision (false unless the
 // tolerance was met).  The reference returns true in that case (bicgstab.h:245-249).
 // The reference's unused host buffer some_vec (bicgstab.h:110-113) is not kept.
 #pragma once

@@ -210,7 +210,7 @@ def limbs_to_fraction(limbs):
 
 
 def ori_bound(n, x, y, u=2.0 ** -53):
-    """North-star acceptance bound u|x.y| + gamma_{n-1}^2 |x|.|y| (floats, generous)."""
+    """Acceptance bound u|x.y| + gamma_{n-1}^2 |x|.|y| of a compensated dot (floats, generous)."""
     g = (n - 1) * u / (1 - (n - 1) * u)
     ad = float(np.dot(np.abs(x), np.abs(y)))
     return g * g * ad
