@@ -36,8 +36,8 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
     if (skip != nullptr && *skip) return;   // solver graphs: flag set, launch does nothing
     if (dalpha != nullptr) alpha = *dalpha;  // solver graphs: scale factor from device memory
     const int lane = threadIdx.x % LPR;
-    const long row_stride = (long)gridDim.x * (SPMV_BLOCK / LPR);
-    for (long r = ((long)blockIdx.x * SPMV_BLOCK + threadIdx.x) / LPR; r < rows; r += row_stride) {
+    const dev::xcd_rows R = dev::xcd_rows::of(rows, SPMV_BLOCK / LPR);
+    for (long r = R.first + (long)threadIdx.x / LPR; r < R.end; r += R.stride) {
         const int beg = rp[r];
         const int end = rp[r + 1];
         T s = T(0), c = T(0);

@@ -182,5 +182,36 @@ __device__ __forceinline__ void store_nt<float4>(float4* p, const float4& v)
     __builtin_nontemporal_store(v.w, &p->w);
 }
 
+// XCD-aware row ranges for the row kernels (SpMV, its fused forms).  Consecutive
+// workgroups are dispatched round-robin over the XCD_COUNT XCDs of the device, each
+// with its own L2; workgroup b runs on XCD b % XCD_COUNT.  Giving each XCD one
+// contiguous slice of the rows keeps the x gathers of neighbouring rows (stencil
+// planes apart) in the same L2.  With a grid that is not a multiple of XCD_COUNT the
+// rows are strided over all blocks in order.  Every row is computed the same way in
+// either mapping.
+constexpr int XCD_COUNT = 8;   // MI355X: 8 XCDs of 32 CUs
+struct xcd_rows {
+    long first, end, stride;   // rows first + k * stride < end of this block (per row slot 0)
+    __device__ static xcd_rows of(long rows, int rows_per_block)
+    {
+        const long G = gridDim.x, b = blockIdx.x;
+        xcd_rows R;
+        if (G % XCD_COUNT != 0 || G < XCD_COUNT) {
+            R.first = b * rows_per_block;
+            R.end = rows;
+            R.stride = G * rows_per_block;
+            return R;
+        }
+        const long gx = G / XCD_COUNT, x = b % XCD_COUNT, local = b / XCD_COUNT;
+        // slices of whole blocks' worth of rows
+        const long chunk = ((rows + XCD_COUNT - 1) / XCD_COUNT + rows_per_block - 1) / rows_per_block * rows_per_block;
+        const long lo = x * chunk;
+        R.end = lo + chunk < rows ? lo + chunk : rows;
+        R.first = lo + local * rows_per_block;
+        R.stride = gx * rows_per_block;
+        return R;
+    }
+};
+
 }  // namespace dev
 }  // namespace hpb

@@ -750,8 +750,8 @@ __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict_
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) ds[q] = dc[q] = T(0);
     const int lane = threadIdx.x % LPR;
-    const long row_stride = (long)gridDim.x * (DB / LPR);
-    for (long r = ((long)blockIdx.x * DB + threadIdx.x) / LPR; r < rows; r += row_stride) {
+    const dev::xcd_rows R = dev::xcd_rows::of(rows, DB / LPR);
+    for (long r = R.first + (long)threadIdx.x / LPR; r < R.end; r += R.stride) {
         const int beg = rp[r], end = rp[r + 1];
         const T rtr = (NQ >= 1 && lane == 0) ? rt[r] : T(0);   // issued with the row bounds
         T s = T(0), c = T(0);
