@@ -20,11 +20,15 @@ The same JSON line carries:
     spmv           256^3 7-point Poisson, plain and compensated rows, GB/s
     krylov         128^3 upwind convection-diffusion (Peclet 100): GMRES(50), BiCGStab(4),
                    naive vs compensated dots, to 1e-12 true relative residual
-    mixed          the same matrix: FP32-inner refinement vs plain FP64, time to 1e-12
+    mixed          the same matrix and 256^3 anisotropic diffusion: FP32-inner GMRES(30) /
+                   BiCGStab(2) with FP64 refinement vs the same solvers in FP64, time to 1e-12
     multidot       fused compensated multi-dot, k = 1..64, GB/s
     batch          512 2-D diffusion systems (sides 64..1024), batched BiCGStab(2) to 1e-10
+    next           MatrixMarket round trip, triangular solves, ILU(0), preconditioned and
+                   plain BiCGStab, complex Dot2
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--headline-only]
+                       [--suite a,b] [--op dot2|copy|nrm2c|exact]
 """
 import argparse
 import ctypes
@@ -466,9 +470,9 @@ def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
         log("mixed precision 256^3 anisotropic diffusion")
         res["aniso256"] = _mixed_on(
             H, h, S, log, 256, S.ANISO7, 2,
-            {"gmres50_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=50, **G),
+            {"gmres30_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=30, **G),
              "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
-            {"gmres50_fp32_refined": S.params(rel_tol=1e-2, max_iters=5000, restart=50, compensated_spmv=1, **G),
+            {"gmres30_fp32_refined": S.params(rel_tol=1e-2, max_iters=5000, restart=30, compensated_spmv=1, **G),
              "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, **B)})
         res["aniso256"]["matrix"] = "256^3 7-point anisotropic diffusion, k_z / k_x = 1e4, Dirichlet"
     return res

@@ -4,7 +4,10 @@ The reference fixes BLOCK_SIZE = 1024 and two elements per thread at compile
 time (source/common/ogita/gpu_reduction_ogita.h:36, gpu_reduction_ogita_impl.cuh:683-694).
 Here the shape is a runtime property of the handle (hpb_set_tuning): threads per
 block, 16-byte vectors per thread per tile, the cap on the grid, and optional
-nontemporal loads (and stores, for the elementwise kernels).  This module times candidate shapes with HIP events on the
+nontemporal loads (and stores, for the elementwise kernels).  For the fused
+multi-dot of Gram-Schmidt (kind "multi") the second field is the number of basis
+columns per chunk (4 or 8) and the grid is counted over all chunks; it is timed
+with 16 basis vectors of length min(n, 2^24).  This module times candidate shapes with HIP events on the
 handle's stream, keeps the fastest per (kind, dtype) and stores the table as
 JSON; ``apply`` loads a table into a handle.
 
@@ -29,7 +32,9 @@ from . import DeviceArray, Handle, call, lib
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "tuning_gfx950.json")
 
-KINDS = {"reduce": 0, "map": 1, "reduce1": 2}
+KINDS = {"reduce": 0, "map": 1, "reduce1": 2, "multi": 3}
+MULTI_K = 16              # basis vectors of the multi-dot timed by the tuner
+MULTI_N_MAX = 1 << 24     # its vector length (at most)
 DTYPES = {"f64": np.float64, "f32": np.float32}
 
 
@@ -38,6 +43,10 @@ MAX_GRID = 4096   # hpb_set_tuning's limit on grid_max
 
 def candidates(kind, num_cu):
     """Launch shapes (block, unroll, grid_max, flags) tried for a kernel family."""
+    if kind == "multi":
+        # block 256; unroll = basis columns per chunk; grid_max = blocks over all chunks
+        return [(256, kc, min(MAX_GRID, max(1, int(num_cu * m))), f)
+                for kc in (4, 8) for m in (1, 2, 4, 8, 16) for f in (0, 1)]
     mults = (0.5, 0.75, 1, 1.5, 2, 4) if kind in ("reduce", "reduce1") else (1, 2, 4, 8, 16)
     out = []
     for block in (256, 512):
@@ -76,9 +85,14 @@ def tune(h, n, reps=20, dtypes=("f64", "f32"), verbose=True):
         c = ctypes.c_double if dt == np.float64 else ctypes.c_float
         for a, sid in ((x, 0), (y, 1), (z, 2)):
             call("hpb_random_uniform_%s" % dname, h.ptr, n, 7, sid, c(-1), c(1), a.ptr)
-        dout = DeviceArray(8, dt, h)
+        dout = DeviceArray(4 * MULTI_K, dt, h)
+        nm = min(n, MULTI_N_MAX)
+        basis = [DeviceArray(nm, dt, h) for _ in range(MULTI_K)]
+        for i, a in enumerate(basis):
+            call("hpb_random_uniform_%s" % dname, h.ptr, nm, 8, i, c(-1), c(1), a.ptr)
+        ptrs = (ctypes.c_void_p * MULTI_K)(*[a.addr for a in basis])
         saved = {k: h.get_tuning(v, dt) for k, v in KINDS.items()}
-        for kind in ("reduce", "reduce1", "map"):
+        for kind in ("reduce", "reduce1", "map", "multi"):
             best = None
             for cand in candidates(kind, ncu):
                 h.set_tuning(KINDS[kind], dt, *cand)
@@ -90,6 +104,14 @@ def tune(h, n, reps=20, dtypes=("f64", "f32"), verbose=True):
                     fn = getattr(L, "hpb_nrm2c_%s_async" % dname)
                     ms = _time(h, fn, (h.ptr, n, x.ptr, dout.ptr), reps)
                     gbs = isz * n / ms / 1e6
+                elif kind == "multi":
+                    if dname == "f64":
+                        ms = _time(h, L.hpb_multidot_f64_async, (h.ptr, MULTI_K, ptrs, nm, x.ptr, 1, dout.ptr), reps)
+                    else:
+                        outh = np.zeros(4 * MULTI_K, dtype=np.float32)
+                        ms = _time(h, L.hpb_multidot_f32, (h.ptr, MULTI_K, ptrs, nm, x.ptr, 1,
+                                                           outh.ctypes.data_as(ctypes.c_void_p)), reps)
+                    gbs = (MULTI_K + 1) * isz * nm / ms / 1e6
                 else:
                     fn = getattr(L, "hpb_axpy_%s" % dname)
                     ms = _time(h, fn, (h.ptr, n, c(1e-3), x.ptr, z.ptr), reps)
@@ -100,7 +122,7 @@ def tune(h, n, reps=20, dtypes=("f64", "f32"), verbose=True):
                     best = list(cand) + [round(gbs, 1)]
             table["%s_%s" % (kind, dname)] = best
             h.set_tuning(KINDS[kind], dt, *saved[kind])
-        for a in (x, y, z, dout):
+        for a in [x, y, z, dout] + basis:
             a.free()
     table["n"] = n
     table["num_cu"] = ncu

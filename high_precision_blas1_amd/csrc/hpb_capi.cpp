@@ -46,6 +46,9 @@ static void default_tuning(hpb_handle_s* h)
         h->tune_red[d] = {512, 4, (3 * cu) / 4 > 0 ? (3 * cu) / 4 : 1, HPB_TUNE_NONTEMPORAL};
         h->tune_red1[d] = {512, 4, cu, HPB_TUNE_NONTEMPORAL};
         h->tune_map[d] = {256, 1, 2 * cu, HPB_TUNE_NONTEMPORAL};
+        // multi-dot: 8 columns per chunk, 8 blocks per CU over all chunks, nontemporal
+        // loads (k = 16, n = 2^24: 6159 GB/s against 5433 with 4 blocks per CU, cached)
+        h->tune_multi[d] = {256, 8, 8 * cu, HPB_TUNE_NONTEMPORAL};
     }
 }
 
@@ -171,16 +174,19 @@ int hpb_memcpy_d2d(hpb_handle_t h, void* dst, const void* src, size_t bytes)
 int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, int grid_max, int flags)
 {
     HPB_REQUIRE(h, HPB_ERR_INVALID, "hpb_set_tuning: null handle");
-    HPB_REQUIRE(kind >= 0 && kind <= 2, HPB_ERR_INVALID, "hpb_set_tuning: kind must be 0, 1 or 2");
+    HPB_REQUIRE(kind >= 0 && kind <= 3, HPB_ERR_INVALID, "hpb_set_tuning: kind must be 0, 1, 2 or 3");
     HPB_REQUIRE(dtype == 0 || dtype == 1, HPB_ERR_INVALID, "hpb_set_tuning: dtype must be 0 or 1");
     HPB_REQUIRE(block == 256 || block == 512, HPB_ERR_UNSUPPORTED, "hpb_set_tuning: block must be 256 or 512");
     HPB_REQUIRE(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, HPB_ERR_UNSUPPORTED,
                 "hpb_set_tuning: unroll must be 1, 2, 4 or 8");
     HPB_REQUIRE(grid_max >= 1 && grid_max <= HPB_MAX_GRID, HPB_ERR_INVALID, "hpb_set_tuning: grid_max out of range");
     HPB_REQUIRE((flags & ~HPB_TUNE_NONTEMPORAL) == 0, HPB_ERR_UNSUPPORTED, "hpb_set_tuning: unknown flags");
+    HPB_REQUIRE(kind != 3 || (block == 256 && (unroll == 4 || unroll == 8)), HPB_ERR_UNSUPPORTED,
+                "hpb_set_tuning: the multi-dot takes block 256 and 4 or 8 columns per chunk");
     hpb_tuning_t t = {block, unroll, grid_max, flags};
     if (kind == 0) h->tune_red[dtype] = t;
     else if (kind == 2) h->tune_red1[dtype] = t;
+    else if (kind == 3) h->tune_multi[dtype] = t;
     else h->tune_map[dtype] = t;
     return HPB_OK;
 }
@@ -188,8 +194,11 @@ int hpb_set_tuning(hpb_handle_t h, int kind, int dtype, int block, int unroll, i
 int hpb_get_tuning(hpb_handle_t h, int kind, int dtype, int* block, int* unroll, int* grid_max, int* flags)
 {
     HPB_REQUIRE(h && block && unroll && grid_max, HPB_ERR_INVALID, "hpb_get_tuning: null argument");
-    HPB_REQUIRE((kind >= 0 && kind <= 2) && (dtype == 0 || dtype == 1), HPB_ERR_INVALID, "hpb_get_tuning: bad kind/dtype");
-    const hpb_tuning_t& t = kind == 0 ? h->tune_red[dtype] : (kind == 2 ? h->tune_red1[dtype] : h->tune_map[dtype]);
+    HPB_REQUIRE((kind >= 0 && kind <= 3) && (dtype == 0 || dtype == 1), HPB_ERR_INVALID, "hpb_get_tuning: bad kind/dtype");
+    const hpb_tuning_t& t = kind == 0 ? h->tune_red[dtype]
+                            : kind == 2 ? h->tune_red1[dtype]
+                            : kind == 3 ? h->tune_multi[dtype]
+                                        : h->tune_map[dtype];
     *block = t.block;
     *unroll = t.unroll;
     *grid_max = t.grid_max;

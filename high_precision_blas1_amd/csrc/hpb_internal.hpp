@@ -32,6 +32,7 @@ struct hpb_handle_s {
     hpb_tuning_t tune_red[2];       // two-input reductions (dot), [0] f64, [1] f32
     hpb_tuning_t tune_red1[2];      // one-input reductions (sum, asum, nrm2)
     hpb_tuning_t tune_map[2];       // elementwise kernels
+    hpb_tuning_t tune_multi[2];     // fused multi-dot: unroll = columns per chunk (4 or 8), grid_max = total blocks
 };
 
 #define HPB_TICKETS 4

@@ -32,7 +32,7 @@ struct coef_pack {
     T a[HPB_MAX_MULTIDOT];
 };
 
-template <typename T, int KC, bool COMP, int BLOCK, bool VEC>
+template <typename T, int KC, bool COMP, int BLOCK, bool VEC, bool NT = false>
 __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T> V, const T* __restrict__ w,
                                                     T* __restrict__ part, const int* __restrict__ skip)
 {
@@ -51,11 +51,15 @@ __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T>
     }
     const Vt* wv = reinterpret_cast<const Vt*>(w);
     for (size_t vi = (size_t)blockIdx.x * BLOCK + threadIdx.x; vi < nv; vi += (size_t)G * BLOCK) {
-        const Vt a = wv[vi];
+        Vt a;
+        if constexpr (NT && VEC) a = dev::load_nt<Vt>(wv + vi);
+        else a = wv[vi];
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
             if (j < kc) {
-                const Vt b = reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
+                Vt b;
+                if constexpr (NT && VEC) b = dev::load_nt<Vt>(reinterpret_cast<const Vt*>(V.p[j0 + j]) + vi);
+                else b = reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
                     T av, bv;
@@ -200,31 +204,42 @@ static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, co
         vec = vec && al16(V[j]);
     }
     for (int j = k; j < HPB_MAX_MULTIDOT; ++j) pk.p[j] = nullptr;
-    constexpr int KC = 8;
-    const int nch = (k + KC - 1) / KC;
+    // launch shape from the handle's tuning (kind 3): columns per chunk (4 or 8),
+    // blocks over all chunks, nontemporal loads
+    const hpb_tuning_t& tu = h->tune_multi[sizeof(T) == 8 ? 0 : 1];
+    const int KCr = tu.unroll == 4 ? 4 : 8;
+    const bool nt = (tu.flags & HPB_TUNE_NONTEMPORAL) != 0;
+    const int nch = (k + KCr - 1) / KCr;
     const size_t W = vec ? dev::vec16<T>::W : 1;
     const size_t nv = n / W;
-    // about 4 blocks per CU in total over the column chunks (at least one per
-    // CU per chunk): enough loads in flight, few partials to merge
     size_t g = (nv + 255) / 256;
     const size_t cus = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
-    size_t cap = (4 * cus) / (size_t)nch;
-    if (cap < cus) cap = cus;
+    size_t cap = (size_t)tu.grid_max / (size_t)nch;
+    if (cap < cus) cap = cus;   // at least one block per CU per chunk
     if (cap * (size_t)k > (size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) cap = ((size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) / k;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
     T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
     T* part = reinterpret_cast<T*>(h->part);
     dim3 grid((unsigned)g, (unsigned)nch);
+#define HPB_MDOT(KC_, C_, V_, NT_)                                                                          \
+    hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip)
+#define HPB_MDOT_KC(C_, V_, NT_)                  \
+    do {                                          \
+        if (KCr == 4) HPB_MDOT(4, C_, V_, NT_);   \
+        else HPB_MDOT(8, C_, V_, NT_);            \
+    } while (0)
     if (comp) {
-        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, true, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
-        else hipLaunchKernelGGL((k_multidot<T, KC, true, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
+        if (vec) { if (nt) HPB_MDOT_KC(true, true, true); else HPB_MDOT_KC(true, true, false); }
+        else HPB_MDOT_KC(true, false, false);
         hipLaunchKernelGGL((k_multidot_fin<T, true>), dim3(1), dim3(1024), 0, h->stream, part, (unsigned)g, k, out, skip);
     } else {
-        if (vec) hipLaunchKernelGGL((k_multidot<T, KC, false, 256, true>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
-        else hipLaunchKernelGGL((k_multidot<T, KC, false, 256, false>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip);
+        if (vec) { if (nt) HPB_MDOT_KC(false, true, true); else HPB_MDOT_KC(false, true, false); }
+        else HPB_MDOT_KC(false, false, false);
         hipLaunchKernelGGL((k_multidot_fin<T, false>), dim3(1), dim3(1024), 0, h->stream, part, (unsigned)g, k, out, skip);
     }
+#undef HPB_MDOT_KC
+#undef HPB_MDOT
     HPB_LAUNCH_CHECK();
     if (!dev_out) {
         HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T) * k, hipMemcpyDeviceToHost, h->stream));

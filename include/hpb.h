@@ -70,7 +70,8 @@ int hpb_memcpy_d2d(hpb_handle_t h, void* dev_dst, const void* dev_src, size_t by
 
 /* ---- tuning / introspection -------------------------------------------- */
 /* kind: 0 two-input reductions (dot), 1 elementwise, 2 one-input reductions
- * (sum, asum, nrm2); dtype: 0 f64, 1 f32.  The autotuner
+ * (sum, asum, nrm2), 3 fused multi-dot (block 256; unroll = basis columns per chunk,
+ * 4 or 8; grid_max = blocks over all chunks); dtype: 0 f64, 1 f32.  The autotuner
  * (high_precision_blas1_amd/autotune.py) writes these; BLOCK_SIZE=1024 is a fixed
  * template parameter in the reference (gpu_reduction_ogita.h:36). */
 #define HPB_TUNE_NONTEMPORAL 1 /* nontemporal loads (and, elementwise, stores) for the streamed tiles */

@@ -13,21 +13,23 @@ import high_precision_blas1_amd.autotune as AT
 import oracle as O
 from conftest import say
 
-KEYS = ["%s_%s" % (k, d) for k in ("reduce", "reduce1", "map") for d in ("f64", "f32")]
+KEYS = ["%s_%s" % (k, d) for k in ("reduce", "reduce1", "map", "multi") for d in ("f64", "f32")]
 
 
-def _valid(block, unroll, grid, flags):
+def _valid(block, unroll, grid, flags, kind=None):
     # the rules of hpb_set_tuning (hpb_capi.cpp)
+    if kind == "multi" and not (block == 256 and unroll in (4, 8)):
+        return False
     return block in (256, 512) and unroll in (1, 2, 4, 8) and 1 <= grid <= 4096 and flags in (0, 1)
 
 
-@pytest.mark.parametrize("kind", ["reduce", "reduce1", "map"])
+@pytest.mark.parametrize("kind", ["reduce", "reduce1", "map", "multi"])
 @pytest.mark.parametrize("ncu", [1, 80, 256, 304])
 def test_candidates_are_valid(kind, ncu):
     c = AT.candidates(kind, ncu)
     assert len(c) == len(set(c)) > 0
     for cand in c:
-        assert _valid(*cand), cand
+        assert _valid(*cand, kind=kind), cand
 
 
 def test_shipped_table():
@@ -36,7 +38,7 @@ def test_shipped_table():
     assert t["num_cu"] == 256 and t["n"] == 1 << 27
     for k in KEYS:
         block, unroll, grid, flags, gbs = t[k]
-        assert _valid(block, unroll, grid, flags), k
+        assert _valid(block, unroll, grid, flags, kind=k.rsplit("_", 1)[0]), k
         assert 0 < gbs < 8000.0, k   # below the HBM peak
 
 
@@ -55,11 +57,11 @@ class _FakeHandle:
 def test_apply_loads_every_entry(tmp_path):
     h = _FakeHandle(256)
     t = AT.apply(h)
-    assert t is not None and len(h.calls) == 6
+    assert t is not None and len(h.calls) == len(KEYS)
     with open(AT.DEFAULT_TABLE) as f:
         ref = json.load(f)
     for kind, dname, block, unroll, grid, flags in h.calls:
-        name = {0: "reduce", 1: "map", 2: "reduce1"}[kind] + "_" + {"float64": "f64", "float32": "f32"}[dname]
+        name = {0: "reduce", 1: "map", 2: "reduce1", 3: "multi"}[kind] + "_" + {"float64": "f64", "float32": "f32"}[dname]
         assert [block, unroll, grid, flags] == ref[name][:4]
     # a table measured on another CU count is not applied
     other = dict(ref, num_cu=304)
@@ -76,7 +78,7 @@ def test_tune_then_parity(handle):
     say("autotune n=%d" % n)
     table = AT.tune(handle, n, reps=2, verbose=False)   # restores the handle's shapes
     for k in KEYS:
-        assert _valid(*table[k][:4]) and table[k][4] > 0
+        assert _valid(*table[k][:4], kind=k.rsplit("_", 1)[0]) and table[k][4] > 0
     h2 = H.Handle()
     for k in KEYS:
         kind, dname = k.rsplit("_", 1)

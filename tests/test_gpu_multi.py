@@ -52,6 +52,35 @@ def test_multidot(handle, k, n):
                 assert abs(res - ex[0]) <= n * U * ad * 1.01 + 1e-300
 
 
+@pytest.mark.parametrize("kc,grid,nt", [(4, 256, 0), (4, 4096, 1), (8, 512, 1), (8, 4096, 0)])
+def test_multidot_tuned_shapes(handle, kc, grid, nt):
+    """Every launch shape the autotuner may pick (kind 3): columns per chunk, grid,
+    nontemporal loads -- the compensated results stay within the ORO bound."""
+    n, k = 300007, 13
+    say("multidot shape", kc, grid, nt)
+    r = np.random.default_rng(kc + grid + nt)
+    w = r.standard_normal(n)
+    Vh = [r.standard_normal(n) for _ in range(k)]
+    dw = H.DeviceArray.from_numpy(w, handle)
+    dV = [H.DeviceArray.from_numpy(v, handle) for v in Vh]
+    saved = handle.get_tuning(3, np.float64)
+    try:
+        handle.set_tuning(3, np.float64, 256, kc, grid, nt)
+        out = np.zeros(4 * k)
+        H.call("hpb_multidot_f64", handle.ptr, k, _ptrs(dV), n, dw.ptr, 1, out.ctypes.data_as(ctypes.c_void_p))
+    finally:
+        handle.set_tuning(3, np.float64, *saved)
+    g = (n - 1) * U / (1 - (n - 1) * U)
+    for j in range(k):
+        ex = O.exact_dot(Vh[j], w)
+        b = U * abs(ex[0]) + g * g * float(np.dot(np.abs(Vh[j]), np.abs(w))) * 1.001
+        assert abs(F(out[4 * j + 2]) - (F(ex[0]) + F(ex[1]))) <= F(b), j
+    with pytest.raises(H.HPBError):
+        handle.set_tuning(3, np.float64, 512, 8, grid, 0)
+    with pytest.raises(H.HPBError):
+        handle.set_tuning(3, np.float64, 256, 2, grid, 0)
+
+
 def test_multiaxpy(handle):
     n, k = 301, 11
     r = np.random.default_rng(4)
