@@ -10,9 +10,11 @@
 //   * per-system scalar recurrences on the device (one lane per system);
 //   * updates and SpMV over all systems in one launch each; tiles of converged
 //     systems return at once;
-//   * the whole cycle (5 SpMV incl. the true-residual recomputation, 6 dot groups,
-//     5 update kernels, 6 scalar kernels) is captured once as a hipGraph and
-//     replayed; the host reads the count of active systems every few cycles.
+//   * every dot product of the cycle is an epilogue of the SpMV that makes its
+//     new vector, so no pass reads a fresh vector back only to take its dots;
+//   * the whole cycle (5 SpMV incl. the true-residual recomputation, 5 update
+//     kernels, 6 scalar kernels) is captured once as a hipGraph and replayed; the
+//     host reads the count of active systems every few cycles.
 // The recurrences are the reference's BiCGStab(L) for L = 2 (bicgstabl.h:199-258)
 // with the MR part evaluated from one fused Gram matrix pass: tau12 = (r2,r1)/s1,
 // s2 = (r2,r2) - 2 tau12 (r2,r1) + tau12^2 s1, which equals the modified
@@ -153,21 +155,53 @@ __global__ __launch_bounds__(256) void k_batch_build(int nsys, int64_t nrows, co
 // y = A D x (MODE 0) or y = b - A D x (MODE 1, compensated rows) for active
 // systems; D = diag(dinv) (Jacobi right preconditioning) or I when dinv is null.
 // The grid strides over the list of tiles of active systems.
-template <int MODE>
+// EPI adds the dot products the cycle takes of the new vector, as per-tile
+// compensated partials in the layout of k_bdots (slot q of tile ti at
+// part[(ti * BQ + q) * 2]), so that no second pass reads y back:
+//   EPI 0: none;
+//   EPI 1: (y, rt) in slot 0;
+//   EPI 2: (y, y) in slot 0 and (y, rt) in slot 1 (the true residual: its norm for
+//          the convergence check, and rho for the next cycle's first BiCG step);
+//   EPI 5: the Gram entries of the MR step with y = r2, x = r1 and z = r0:
+//          (r1, r1), (r2, r1), (r2, r2), (r0, r1), (r0, r2) in slots 0..4.
+template <int MODE, int EPI>
 __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ tiles, const int* __restrict__ list,
                                                       const int* __restrict__ count, const int* __restrict__ active,
                                                       const int* __restrict__ rp, const int* __restrict__ col,
                                                       const double* __restrict__ val, const double* __restrict__ dinv,
                                                       const double* __restrict__ x, const double* __restrict__ b,
-                                                      double* __restrict__ y)
+                                                      double* __restrict__ y, const double* __restrict__ rt,
+                                                      const double* __restrict__ z, double* __restrict__ part)
 {
+    static_assert(EPI == 0 || EPI == 1 || EPI == 2 || EPI == 5, "k_bspmv: epilogue");
+    constexpr int NQ = EPI == 0 ? 1 : EPI;
+    constexpr int NW = BT_THREADS / 64;
+    // wave totals of the tile partials; two buffers, alternating by tile, so that one
+    // barrier per tile suffices: buffer k is read after the barrier of its tile and
+    // written again only two tiles later, after every thread passed the next barrier
+    __shared__ double wtot[2][NQ][NW][2];
+    int buf = 0;
     const int nt = *count;
     for (int q = blockIdx.x; q < nt; q += gridDim.x) {
-        const btile t = tiles[list[q]];
+        const int ti = list[q];
+        const btile t = tiles[ti];
         if (!active[t.sys]) continue;
         constexpr int LPR = 4;
         const int lane = threadIdx.x % LPR;
+        double ds[NQ], dc[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) ds[k] = dc[k] = 0.0;
         for (int64_t r = t.r0 + threadIdx.x / LPR; r < t.r1; r += BT_THREADS / LPR) {
+            // the epilogue's own operands, loaded ahead of the row's gathers
+            double e0 = 0.0, e1 = 0.0;
+            if (EPI != 0 && lane == 0) {
+                if (EPI == 5) {
+                    e0 = x[r];
+                    e1 = z[r];
+                } else {
+                    e0 = rt[r];
+                }
+            }
             double s = (MODE == 1 && lane == 0) ? b[r] : 0.0, c = 0.0;
             for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
                 const double a = MODE == 1 ? -val[k] : val[k];
@@ -180,7 +214,43 @@ __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ 
                 double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
                 dev::merge<double>(s, c, s2, c2);
             }
-            if (lane == 0) y[r] = s + c;
+            if (lane == 0) {
+                const double yv = s + c;
+                y[r] = yv;
+                if (EPI == 1) {
+                    dev::dot2_step<double>(ds[0], dc[0], yv, e0);
+                } else if (EPI == 2) {
+                    dev::dot2_step<double>(ds[0], dc[0], yv, yv);
+                    dev::dot2_step<double>(ds[NQ - 1], dc[NQ - 1], yv, e0);
+                } else if (EPI == 5) {   // e0 = r1, e1 = r0, yv = r2
+                    dev::dot2_step<double>(ds[0], dc[0], e0, e0);
+                    dev::dot2_step<double>(ds[1 % NQ], dc[1 % NQ], yv, e0);
+                    dev::dot2_step<double>(ds[2 % NQ], dc[2 % NQ], yv, yv);
+                    dev::dot2_step<double>(ds[3 % NQ], dc[3 % NQ], e1, e0);
+                    dev::dot2_step<double>(ds[4 % NQ], dc[4 % NQ], e1, yv);
+                }
+            }
+        }
+        if (EPI != 0) {
+            const int w = threadIdx.x >> 6;
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) {
+                dev::wave_reduce<double, true>(ds[k], dc[k]);
+                if ((threadIdx.x & 63) == 0) {
+                    wtot[buf][k][w][0] = ds[k];
+                    wtot[buf][k][w][1] = dc[k];
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x < NQ) {   // waves merged in wave order, as block_reduce does
+                const int k = threadIdx.x;
+                double fs = wtot[buf][k][0][0], fc = wtot[buf][k][0][1];
+#pragma unroll
+                for (int v = 1; v < NW; ++v) dev::merge<double>(fs, fc, wtot[buf][k][v][0], wtot[buf][k][v][1]);
+                part[((size_t)ti * BQ + k) * 2] = fs;
+                part[((size_t)ti * BQ + k) * 2 + 1] = fc;
+            }
+            buf ^= 1;
         }
     }
 }
@@ -219,22 +289,24 @@ __global__ __launch_bounds__(BT_THREADS) void k_bdots(const btile* __restrict__ 
     }
 }
 
-// one wave per system: merge the system's tile partials in tile order
-template <int NQ>
+// one wave per system: merge the system's tile partials of slots Q0 .. Q0 + NQ - 1
+// in tile order
+template <int NQ, int Q0>
 __device__ __forceinline__ void merge_sys(int s, const int* tile_first, const double* part, double* d)
 {
     const int lane = threadIdx.x & 63;
     for (int q = 0; q < NQ; ++q) {
         double fs = 0.0, fc = 0.0;
         for (int t = tile_first[s] + lane; t < tile_first[s + 1]; t += 64)
-            dev::merge<double>(fs, fc, part[((size_t)t * BQ + q) * 2], part[((size_t)t * BQ + q) * 2 + 1]);
+            dev::merge<double>(fs, fc, part[((size_t)t * BQ + Q0 + q) * 2], part[((size_t)t * BQ + Q0 + q) * 2 + 1]);
         dev::wave_reduce<double, true>(fs, fc);
         d[q] = fs + fc;
     }
 }
 
-// phase: 0 rho1 -> beta; 1 alpha; 2 MR; 3 residual check; 4 init (rnorm, bnorm)
-template <int NQ, int PHASE>
+// phase: 0 rho1 -> beta; 1 alpha; 2 MR; 3 residual check; 4 init (rnorm, bnorm);
+// the NQ dot products come from slots Q0 ..
+template <int NQ, int PHASE, int Q0 = 0>
 __global__ __launch_bounds__(256) void k_bsys(int nsys, const int* __restrict__ tile_first,
                                               const double* __restrict__ part, double* __restrict__ st,
                                               int* __restrict__ active, int* __restrict__ iters, int* __restrict__ nactive,
@@ -242,8 +314,9 @@ __global__ __launch_bounds__(256) void k_bsys(int nsys, const int* __restrict__ 
 {
     const int s = (blockIdx.x * 256 + threadIdx.x) >> 6;
     if (s >= nsys || !active[s]) return;
+    static_assert(Q0 + NQ <= BQ, "k_bsys: slots");
     double d[NQ];
-    merge_sys<NQ>(s, tile_first, part, d);
+    merge_sys<NQ, Q0>(s, tile_first, part, d);
     if ((threadIdx.x & 63) != 0) return;
     double* S = st + (size_t)s * S_NSTATE;
     if (PHASE == 0) {  // d0 = (r_j, rt)
@@ -404,32 +477,31 @@ static int free_batch(batch_t* B)
 
 static unsigned sys_blocks(int nsys) { return (unsigned)((nsys * 64 + 255) / 256); }
 
-// record one BiCGStab(2) cycle on the batch's stream
+// record one BiCGStab(2) cycle on the batch's stream.  Every dot product of the
+// cycle is an epilogue of the SpMV that makes its new vector; the cycle's first
+// (r0, r~) comes from slot 1 of the previous cycle's true-residual pass (of the
+// set-up pass, where r~ = r0, on the first cycle).
 static void record_cycle(batch_t* B, double tol2)
 {
     hipStream_t st = B->h->stream;
     const unsigned G = (unsigned)B->grid, SB = sys_blocks(B->nsys);
     double** v = B->v;
     const double* dinv = B->jacobi ? B->dinv : nullptr;
-    auto dots1 = [&](const double* a, const double* b) {
-        dot_pairs P{};
-        P.a[0] = a;
-        P.b[0] = b;
-        hipLaunchKernelGGL((k_bdots<1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
-                           B->active, P, B->part);
-    };
-    auto spmv = [&](const double* x, double* y) {
-        hipLaunchKernelGGL((k_bspmv<0>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
-                           B->active, B->rp, B->col, B->val, dinv, x, (const double*)nullptr, y);
+    const double* none = nullptr;
+    auto spmv_rt = [&](const double* x, double* y) {   // y = A D x and (y, r~)
+        hipLaunchKernelGGL((k_bspmv<0, 1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list,
+                           B->tile_count, B->active, B->rp, B->col, B->val, dinv, x, none, y, (const double*)v[VRT],
+                           none, B->part);
     };
     auto upd = [&](auto kind) {
         hipLaunchKernelGGL((k_bupd<decltype(kind)::value>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles,
                            B->tile_list, B->tile_count, B->active, B->st, v[VX], v[VR0], v[VR1], v[VR2], v[VU0],
                            v[VU1], v[VU2]);
     };
-    auto sys = [&](auto nq, auto phase, int first) {
-        hipLaunchKernelGGL((k_bsys<decltype(nq)::value, decltype(phase)::value>), dim3(SB), dim3(256), 0, st, B->nsys,
-                           B->d_tile_first, B->part, B->st, B->active, B->iters, B->nactive, tol2, first);
+    auto sys = [&](auto nq, auto phase, auto q0, int first) {
+        hipLaunchKernelGGL((k_bsys<decltype(nq)::value, decltype(phase)::value, decltype(q0)::value>), dim3(SB),
+                           dim3(256), 0, st, B->nsys, B->d_tile_first, B->part, B->st, B->active, B->iters,
+                           B->nactive, tol2, first);
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
@@ -439,41 +511,30 @@ static void record_cycle(batch_t* B, double tol2)
     using I5 = std::integral_constant<int, 5>;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, B->nactive);
     // j = 0
-    dots1(v[VR0], v[VRT]);
-    sys(I1{}, I0{}, 1);
+    sys(I1{}, I0{}, I1{}, 1);        // rho1 = (r0, r~), beta
     upd(I0{});                       // u0 = r0 - beta u0
-    spmv(v[VU0], v[VU1]);            // u1 = A D u0
-    dots1(v[VU1], v[VRT]);
-    sys(I1{}, I1{}, 0);              // alpha
+    spmv_rt(v[VU0], v[VU1]);         // u1 = A D u0, (u1, r~)
+    sys(I1{}, I1{}, I0{}, 0);        // alpha
     upd(I1{});                       // r0 -= alpha u1, x += alpha u0
-    spmv(v[VR0], v[VR1]);            // r1 = A D r0
+    spmv_rt(v[VR0], v[VR1]);         // r1 = A D r0, (r1, r~)
     // j = 1
-    dots1(v[VR1], v[VRT]);
-    sys(I1{}, I0{}, 0);
+    sys(I1{}, I0{}, I0{}, 0);
     upd(I2{});                       // u0, u1
-    spmv(v[VU1], v[VU2]);            // u2 = A D u1
-    dots1(v[VU2], v[VRT]);
-    sys(I1{}, I1{}, 0);
+    spmv_rt(v[VU1], v[VU2]);         // u2 = A D u1, (u2, r~)
+    sys(I1{}, I1{}, I0{}, 0);
     upd(I3{});                       // r0, r1, x
-    spmv(v[VR1], v[VR2]);            // r2 = A D r1
-    // MR part
-    {
-        dot_pairs P{};
-        P.a[0] = v[VR1]; P.b[0] = v[VR1];
-        P.a[1] = v[VR2]; P.b[1] = v[VR1];
-        P.a[2] = v[VR2]; P.b[2] = v[VR2];
-        P.a[3] = v[VR0]; P.b[3] = v[VR1];
-        P.a[4] = v[VR0]; P.b[4] = v[VR2];
-        hipLaunchKernelGGL((k_bdots<5>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
-                           B->active, P, B->part);
-    }
-    sys(I5{}, I2{}, 0);
+    // MR part: r2 = A D r1 and the Gram entries of r0, r1, r2
+    hipLaunchKernelGGL((k_bspmv<0, 5>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                       B->active, B->rp, B->col, B->val, dinv, (const double*)v[VR1], none, v[VR2], none,
+                       (const double*)v[VR0], B->part);
+    sys(I5{}, I2{}, I0{}, 0);
     upd(I4{});
-    // true residual r0 = b - A D x and the convergence check
-    hipLaunchKernelGGL((k_bspmv<1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
-                       B->active, B->rp, B->col, B->val, dinv, v[VX], v[VB], v[VR0]);
-    dots1(v[VR0], v[VR0]);
-    sys(I1{}, I3{}, 0);
+    // true residual r0 = b - A D x, (r0, r0) for the convergence check, (r0, r~)
+    // for the next cycle
+    hipLaunchKernelGGL((k_bspmv<1, 2>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                       B->active, B->rp, B->col, B->val, dinv, (const double*)v[VX], (const double*)v[VB], v[VR0],
+                       (const double*)v[VRT], none, B->part);
+    sys(I1{}, I3{}, I0{}, 0);
     // next cycle only visits the tiles of systems still active
     hipLaunchKernelGGL(k_bcompact, dim3(1), dim3(256), 0, st, B->nsys, B->d_tile_first, B->active, B->tile_list,
                        B->tile_count);
@@ -633,6 +694,8 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     HPB_HIP(hipMemsetAsync(B->iters, 0, sizeof(int) * B->nsys, st));
     if (int rc = all_active()) return rc;
     {
+        // slot 0: (b, b); slot 1: (r0, r0), which is also the (r0, r~) the first
+        // cycle starts from
         dot_pairs P{};
         P.a[0] = v[VB];
         P.b[0] = v[VB];
@@ -693,8 +756,9 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // per-system true relative residuals ||b - A x|| / ||b|| of the original systems, compensated
     if (int rc = all_active()) return rc;
-    hipLaunchKernelGGL((k_bspmv<1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
-                       B->active, B->rp, B->col, B->val, (const double*)nullptr, v[VX], v[VB], v[VR1]);
+    hipLaunchKernelGGL((k_bspmv<1, 0>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
+                       B->active, B->rp, B->col, B->val, (const double*)nullptr, (const double*)v[VX],
+                       (const double*)v[VB], v[VR1], (const double*)nullptr, (const double*)nullptr, B->part);
     {
         dot_pairs P{};
         P.a[0] = v[VR1];
