@@ -14,7 +14,12 @@
 //     new vector, so no pass reads a fresh vector back only to take its dots;
 //   * the whole cycle (5 SpMV incl. the true-residual recomputation, 5 update
 //     kernels, 6 scalar kernels) is captured once as a hipGraph and replayed; the
-//     host reads the count of active systems every few cycles.
+//     host reads the count of active systems every few cycles;
+//   * by default (resid_recalc_freq 0, the reference's default) the cycles test
+//     the recursive residual and compute the true one only for the systems that
+//     claim convergence; with resid_recalc_freq k every k-th cycle replaces the
+//     recursive residual by the true one for all systems (a second captured
+//     graph, chosen per cycle by the host).
 // The recurrences are the reference's BiCGStab(L) for L = 2 (bicgstabl.h:199-258)
 // with the MR part evaluated from one fused Gram matrix pass: tau12 = (r2,r1)/s1,
 // s2 = (r2,r2) - 2 tau12 (r2,r1) + tau12^2 s1, which equals the modified
@@ -78,12 +83,16 @@ struct batch_t {
     double* dinv = nullptr;         // nrows: 1 / a_ii (Jacobi right preconditioning)
     double* vald = nullptr;         // nnz: the values of A D (columns scaled by dinv), made on first use
     int jacobi = 0;
-    double graph_tol2 = -1.0;       // settings the captured graph was recorded with
+    int* confirm = nullptr;         // nsys: the recursive residual met the tolerance this cycle
+    int recalc = 0;                 // true residual b - A x every recalc cycles (0: only to confirm)
+    double graph_tol2 = -1.0;       // settings the captured graphs were recorded with
     int graph_jacobi = -1;
     int grid = 1;                   // blocks of the SpMV and dot kernels (over tile_list)
     int ugrid = 1;                  // blocks of the update kernels
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;     // cycle that recomputes the true residual
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph2 = nullptr;    // cycle on the recursive residual
+    hipGraphExec_t exec2 = nullptr;
     int verbose = 0;                // print the active count at every check
 };
 enum { VB = 0, VX, VR0, VR1, VR2, VU0, VU1, VU2, VRT, VXS };
@@ -171,6 +180,7 @@ __global__ __launch_bounds__(256) void k_batch_build(int nsys, int64_t nrows, co
 //          the convergence check, and rho for the next cycle's first BiCG step);
 //   EPI 5: the Gram entries of the MR step with y = r2, x = r1 and z = r0:
 //          (r1, r1), (r2, r1), (r2, r2), (r0, r1), (r0, r2) in slots 0..4.
+// With only != null the tiles of systems with only[sys] == 0 are skipped.
 template <int MODE, int EPI>
 __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ tiles, const int* __restrict__ list,
                                                       const int* __restrict__ count, const int* __restrict__ active,
@@ -178,7 +188,8 @@ __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ 
                                                       const double* __restrict__ val, const double* __restrict__ x,
                                                       const double* __restrict__ b, double* __restrict__ y,
                                                       const double* __restrict__ rt,
-                                                      const double* __restrict__ z, double* __restrict__ part)
+                                                      const double* __restrict__ z, double* __restrict__ part,
+                                                      const int* __restrict__ only)
 {
     static_assert(EPI == 0 || EPI == 1 || EPI == 2 || EPI == 5, "k_bspmv: epilogue");
     constexpr int NQ = EPI == 0 ? 1 : EPI;
@@ -193,7 +204,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ 
     for (long q = Q.first; q < Q.end; q += Q.stride) {
         const int ti = list[q];
         const btile t = tiles[ti];
-        if (!active[t.sys]) continue;
+        if (!active[t.sys] || (only != nullptr && !only[t.sys])) continue;
         constexpr int LPR = 4;
         const int lane = threadIdx.x % LPR;
         double ds[NQ], dc[NQ];
@@ -311,16 +322,19 @@ __device__ __forceinline__ void merge_sys(int s, const int* tile_first, const do
     }
 }
 
-// phase: 0 rho1 -> beta; 1 alpha; 2 MR; 3 residual check; 4 init (rnorm, bnorm);
-// the NQ dot products come from slots Q0 ..
+// phase: 0 rho1 -> beta; 1 alpha; 2 MR; 3 true-residual check; 4 init (rnorm,
+// bnorm); 5 recursive-residual check: a system that meets the tolerance is marked
+// for confirmation; 6 true-residual check of the marked systems.  The NQ dot
+// products come from slots Q0 ..
 template <int NQ, int PHASE, int Q0 = 0>
 __global__ __launch_bounds__(256) void k_bsys(int nsys, const int* __restrict__ tile_first,
                                               const double* __restrict__ part, double* __restrict__ st,
                                               int* __restrict__ active, int* __restrict__ iters, int* __restrict__ nactive,
-                                              double tol2, int first)
+                                              int* __restrict__ confirm, double tol2, int first)
 {
     const int s = (blockIdx.x * 256 + threadIdx.x) >> 6;
     if (s >= nsys || !active[s]) return;
+    if (PHASE == 6 && !confirm[s]) return;
     static_assert(Q0 + NQ <= BQ, "k_bsys: slots");
     double d[NQ];
     merge_sys<NQ, Q0>(s, tile_first, part, d);
@@ -355,7 +369,13 @@ __global__ __launch_bounds__(256) void k_bsys(int nsys, const int* __restrict__ 
         S[S_GPP1] = g2;
         S[S_OMEGA] = g2;
         iters[s] += 2;
-    } else if (PHASE == 3) {  // d0 = (r0, r0) of the true residual
+    } else if (PHASE == 5) {  // d0 = (r0, r0) of the recursive residual
+        S[S_RNORM] = d[0];
+        if (d[0] <= tol2 * S[S_BNORM]) confirm[s] = 1;   // decided by the true residual, this cycle
+        else if (d[0] == d[0]) atomicAdd(nactive, 1);
+        else active[s] = 0;
+    } else if (PHASE == 3 || PHASE == 6) {  // d0 = (r0, r0) of the true residual
+        if (PHASE == 6) confirm[s] = 0;
         S[S_RNORM] = d[0];
         if (!(d[0] <= tol2 * S[S_BNORM])) {
             if (d[0] == d[0]) atomicAdd(nactive, 1);
@@ -406,21 +426,17 @@ constexpr int upd_reads(int kind)
                      : RD_X | RD_R0 | RD_R1 | RD_R2 | RD_U0 | RD_U1 | RD_U2;
 }
 struct upd_vals {
-    double x, r0, r1, r2, u0, u1, u2;
+    double x, r0, r1, r2, u0, u1, u2, rt;
 };
 
 constexpr int UPD_ROWS = 4;   // rows per thread per step, all loaded before any is written
 
-// KIND 0: u0 = r0 - beta u0; 1: r0 -= alpha u1, x += alpha u0; 2: u0 = r0 - beta u0,
-// u1 = r1 - beta u1; 3: r0 -= alpha u1, r1 -= alpha u2, x += alpha u0; 4: the MR
-// update x += g1 r0 + gpp1 r1, r0 -= gp2 r2 - c1 r1, u0 -= g2 u2 + g1 u1.
-// Rows r + u * BT_THREADS (u < UPD_ROWS) below end: the loads of all of them are
-// issued first, so a wave keeps UPD_ROWS times as many loads in flight.
-template <int KIND>
+template <int KIND, bool DOTS>
 __device__ __forceinline__ void upd_load(int64_t i, upd_vals& v, const double* __restrict__ x,
                                          const double* __restrict__ r0, const double* __restrict__ r1,
                                          const double* __restrict__ r2, const double* __restrict__ u0,
-                                         const double* __restrict__ u1, const double* __restrict__ u2)
+                                         const double* __restrict__ u1, const double* __restrict__ u2,
+                                         const double* __restrict__ rt)
 {
     constexpr int M = upd_reads(KIND);
     if (M & RD_X) v.x = x[i];
@@ -430,13 +446,16 @@ __device__ __forceinline__ void upd_load(int64_t i, upd_vals& v, const double* _
     if (M & RD_U0) v.u0 = u0[i];
     if (M & RD_U1) v.u1 = u1[i];
     if (M & RD_U2) v.u2 = u2[i];
+    if (DOTS) v.rt = rt[i];
 }
 
+// stores row i's update; returns the new r0 (KIND 4)
 template <int KIND>
-__device__ __forceinline__ void upd_store(int64_t i, const upd_vals& a, const upd_coef& k, double* __restrict__ x,
-                                          double* __restrict__ r0, double* __restrict__ r1, double* __restrict__ u0,
-                                          double* __restrict__ u1)
+__device__ __forceinline__ double upd_store(int64_t i, const upd_vals& a, const upd_coef& k, double* __restrict__ x,
+                                            double* __restrict__ r0, double* __restrict__ r1,
+                                            double* __restrict__ u0, double* __restrict__ u1)
 {
+    double rn = 0.0;
     if (KIND == 0) {
         u0[i] = dev::fma_rn<double>(-k.beta, a.u0, a.r0);
     } else if (KIND == 1) {
@@ -451,55 +470,98 @@ __device__ __forceinline__ void upd_store(int64_t i, const upd_vals& a, const up
         x[i] = dev::fma_rn<double>(k.alpha, a.u0, a.x);
     } else {
         x[i] = dev::fma_rn<double>(k.gpp1, a.r1, dev::fma_rn<double>(k.g1, a.r0, a.x));
-        r0[i] = dev::fma_rn<double>(k.c1, a.r1, dev::fma_rn<double>(-k.gp2, a.r2, a.r0));
+        rn = dev::fma_rn<double>(k.c1, a.r1, dev::fma_rn<double>(-k.gp2, a.r2, a.r0));
+        r0[i] = rn;
         u0[i] = dev::fma_rn<double>(-k.g1, a.u1, dev::fma_rn<double>(-k.g2, a.u2, a.u0));
     }
+    return rn;
 }
 
 // KIND 0: u0 = r0 - beta u0; 1: r0 -= alpha u1, x += alpha u0; 2: u0 = r0 - beta u0,
 // u1 = r1 - beta u1; 3: r0 -= alpha u1, r1 -= alpha u2, x += alpha u0; 4: the MR
-// update x += g1 r0 + gpp1 r1, r0 -= gp2 r2 - c1 r1, u0 -= g2 u2 + g1 u1.
+// update x += g1 r0 + gpp1 r1, r0 -= gp2 r2 - c1 r1, u0 -= g2 u2 + g1 u1; with DOTS
+// (KIND 4) also (r0, r0) and (r0, r~) of the new r0 into (ds, dc)[0] and [1].
 // Rows r + u * BT_THREADS, u < UPD_ROWS: when all lie below end their loads are
 // issued together before any store, so a wave keeps UPD_ROWS times as many loads
 // in flight; the rows of a partial step go one at a time.
-template <int KIND>
+template <int KIND, bool DOTS>
 __device__ __forceinline__ void upd_step(int64_t r, int64_t end, const upd_coef& k, double* __restrict__ x,
                                          double* __restrict__ r0, double* __restrict__ r1,
                                          const double* __restrict__ r2, double* __restrict__ u0,
-                                         double* __restrict__ u1, const double* __restrict__ u2)
+                                         double* __restrict__ u1, const double* __restrict__ u2,
+                                         const double* __restrict__ rt, double* ds, double* dc)
 {
+    auto dots = [&](double rn, double rtv) {
+        if (DOTS) {
+            dev::dot2_step<double>(ds[0], dc[0], rn, rn);
+            dev::dot2_step<double>(ds[1], dc[1], rn, rtv);
+        }
+    };
     if (r + (int64_t)(UPD_ROWS - 1) * BT_THREADS < end) {
         upd_vals v[UPD_ROWS];
 #pragma unroll
-        for (int u = 0; u < UPD_ROWS; ++u) upd_load<KIND>(r + (int64_t)u * BT_THREADS, v[u], x, r0, r1, r2, u0, u1, u2);
+        for (int u = 0; u < UPD_ROWS; ++u)
+            upd_load<KIND, DOTS>(r + (int64_t)u * BT_THREADS, v[u], x, r0, r1, r2, u0, u1, u2, rt);
 #pragma unroll
-        for (int u = 0; u < UPD_ROWS; ++u) upd_store<KIND>(r + (int64_t)u * BT_THREADS, v[u], k, x, r0, r1, u0, u1);
+        for (int u = 0; u < UPD_ROWS; ++u)
+            dots(upd_store<KIND>(r + (int64_t)u * BT_THREADS, v[u], k, x, r0, r1, u0, u1), v[u].rt);
     } else {
         for (int64_t i = r; i < end; i += BT_THREADS) {
             upd_vals v;
-            upd_load<KIND>(i, v, x, r0, r1, r2, u0, u1, u2);
-            upd_store<KIND>(i, v, k, x, r0, r1, u0, u1);
+            upd_load<KIND, DOTS>(i, v, x, r0, r1, r2, u0, u1, u2, rt);
+            dots(upd_store<KIND>(i, v, k, x, r0, r1, u0, u1), v.rt);
         }
     }
 }
 
-// updates; every tile reads its system's scalars
-template <int KIND>
+// updates; every tile reads its system's scalars.  DOTS (the MR update of a cycle
+// on the recursive residual): per-tile partials of (r0, r0) and (r0, r~) of the new
+// r0 in slots 0 and 1, as the true-residual SpMV leaves them.
+template <int KIND, bool DOTS = false>
 __global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ tiles, const int* __restrict__ list,
                                                      const int* __restrict__ count, const int* __restrict__ active,
                                                      const double* __restrict__ st, double* __restrict__ x,
                                                      double* __restrict__ r0, double* __restrict__ r1,
                                                      double* __restrict__ r2, double* __restrict__ u0,
-                                                     double* __restrict__ u1, double* __restrict__ u2)
+                                                     double* __restrict__ u1, double* __restrict__ u2,
+                                                     const double* __restrict__ rt, double* __restrict__ part)
 {
+    static_assert(!DOTS || KIND == 4, "k_bupd: dot products of the MR update only");
+    constexpr int NW = BT_THREADS / 64;
+    // wave totals, double-buffered by tile as in k_bspmv: one barrier per tile
+    __shared__ double wtot[2][2][NW][2];
+    int buf = 0;
     const int nt = *count;
     const dev::xcd_rows Q = dev::xcd_rows::of(nt, 1);
     for (long q = Q.first; q < Q.end; q += Q.stride) {
-        const btile t = tiles[list[q]];
+        const int ti = list[q];
+        const btile t = tiles[ti];
         if (!active[t.sys]) continue;
         const upd_coef k = load_coef<KIND>(st + (size_t)t.sys * S_NSTATE);
+        double ds[2] = {0.0, 0.0}, dc[2] = {0.0, 0.0};
         for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += (int64_t)UPD_ROWS * BT_THREADS)
-            upd_step<KIND>(r, t.r1, k, x, r0, r1, r2, u0, u1, u2);
+            upd_step<KIND, DOTS>(r, t.r1, k, x, r0, r1, r2, u0, u1, u2, rt, ds, dc);
+        if (DOTS) {
+            const int w = threadIdx.x >> 6;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                dev::wave_reduce<double, true>(ds[j], dc[j]);
+                if ((threadIdx.x & 63) == 0) {
+                    wtot[buf][j][w][0] = ds[j];
+                    wtot[buf][j][w][1] = dc[j];
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x < 2) {   // waves merged in wave order
+                const int j = threadIdx.x;
+                double fs = wtot[buf][j][0][0], fc = wtot[buf][j][0][1];
+#pragma unroll
+                for (int v = 1; v < NW; ++v) dev::merge<double>(fs, fc, wtot[buf][j][v][0], wtot[buf][j][v][1]);
+                part[((size_t)ti * BQ + j) * 2] = fs;
+                part[((size_t)ti * BQ + j) * 2 + 1] = fc;
+            }
+            buf ^= 1;
+        }
     }
 }
 
@@ -570,8 +632,11 @@ static int free_batch(batch_t* B)
     if (!B) return HPB_OK;
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->exec2) (void)hipGraphExecDestroy(B->exec2);
+    if (B->graph2) (void)hipGraphDestroy(B->graph2);
     void* ptrs[] = {B->d_tiles, B->d_tile_first, B->d_row_off, B->rp, B->col, B->val, B->st, B->active,
-                    B->iters, B->nactive, B->part, B->dots, B->tile_list, B->tile_count, B->dinv, B->vald};
+                    B->iters, B->nactive, B->part, B->dots, B->tile_list, B->tile_count, B->dinv, B->vald,
+                    B->confirm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (double* p : B->v)
@@ -584,10 +649,13 @@ static int free_batch(batch_t* B)
 static unsigned sys_blocks(int nsys) { return (unsigned)((nsys * 64 + 255) / 256); }
 
 // record one BiCGStab(2) cycle on the batch's stream.  Every dot product of the
-// cycle is an epilogue of the SpMV that makes its new vector; the cycle's first
-// (r0, r~) comes from slot 1 of the previous cycle's true-residual pass (of the
-// set-up pass, where r~ = r0, on the first cycle).
-static void record_cycle(batch_t* B, double tol2)
+// cycle is an epilogue of the kernel that makes its new vector; the cycle's first
+// (r0, r~) comes from slot 1 of the previous cycle's last pass (of the set-up pass,
+// where r~ = r0, on the first cycle).  recalc: the cycle ends with the true
+// residual b - A x replacing r0 and the convergence check on it; otherwise the
+// check runs on the recursive r0 of the MR update, and only the systems that meet
+// the tolerance get the true residual (which replaces r0) and the final check.
+static void record_cycle(batch_t* B, double tol2, bool recalc)
 {
     hipStream_t st = B->h->stream;
     const unsigned G = (unsigned)B->grid, GU = (unsigned)B->ugrid, SB = sys_blocks(B->nsys);
@@ -597,17 +665,17 @@ static void record_cycle(batch_t* B, double tol2)
     auto spmv_rt = [&](const double* x, double* y) {   // y = A D x and (y, r~)
         hipLaunchKernelGGL((k_bspmv<0, 1>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list,
                            B->tile_count, B->active, B->rp, B->col, mval, x, none, y, (const double*)v[VRT],
-                           none, B->part);
+                           none, B->part, (const int*)nullptr);
     };
     auto upd = [&](auto kind) {
         hipLaunchKernelGGL((k_bupd<decltype(kind)::value>), dim3(GU), dim3(BT_THREADS), 0, st, B->d_tiles,
                            B->tile_list, B->tile_count, B->active, B->st, v[VX], v[VR0], v[VR1], v[VR2], v[VU0],
-                           v[VU1], v[VU2]);
+                           v[VU1], v[VU2], none, (double*)nullptr);
     };
     auto sys = [&](auto nq, auto phase, auto q0, int first) {
         hipLaunchKernelGGL((k_bsys<decltype(nq)::value, decltype(phase)::value, decltype(q0)::value>), dim3(SB),
                            dim3(256), 0, st, B->nsys, B->d_tile_first, B->part, B->st, B->active, B->iters,
-                           B->nactive, tol2, first);
+                           B->nactive, B->confirm, tol2, first);
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
@@ -632,15 +700,27 @@ static void record_cycle(batch_t* B, double tol2)
     // MR part: r2 = A D r1 and the Gram entries of r0, r1, r2
     hipLaunchKernelGGL((k_bspmv<0, 5>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
                        B->active, B->rp, B->col, mval, (const double*)v[VR1], none, v[VR2], none,
-                       (const double*)v[VR0], B->part);
+                       (const double*)v[VR0], B->part, (const int*)nullptr);
     sys(I5{}, I2{}, I0{}, 0);
-    upd(I4{});
-    // true residual r0 = b - A D x, (r0, r0) for the convergence check, (r0, r~)
-    // for the next cycle
-    hipLaunchKernelGGL((k_bspmv<1, 2>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
-                       B->active, B->rp, B->col, mval, (const double*)v[VX], (const double*)v[VB], v[VR0],
-                       (const double*)v[VRT], none, B->part);
-    sys(I1{}, I3{}, I0{}, 0);
+    // the true residual r0 = b - A D x with (r0, r0) and (r0, r~), for all systems
+    // or for those marked in confirm
+    auto true_resid = [&](const int* only) {
+        hipLaunchKernelGGL((k_bspmv<1, 2>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list,
+                           B->tile_count, B->active, B->rp, B->col, mval, (const double*)v[VX], (const double*)v[VB],
+                           v[VR0], (const double*)v[VRT], none, B->part, only);
+    };
+    if (recalc) {
+        upd(I4{});
+        true_resid(nullptr);
+        sys(I1{}, I3{}, I0{}, 0);
+    } else {
+        hipLaunchKernelGGL((k_bupd<4, true>), dim3(GU), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list,
+                           B->tile_count, B->active, B->st, v[VX], v[VR0], v[VR1], v[VR2], v[VU0], v[VU1], v[VU2],
+                           (const double*)v[VRT], B->part);
+        sys(I1{}, std::integral_constant<int, 5>{}, I0{}, 0);
+        true_resid(B->confirm);
+        sys(I1{}, std::integral_constant<int, 6>{}, I0{}, 0);
+    }
     // next cycle only visits the tiles of systems still active
     hipLaunchKernelGGL(k_bcompact, dim3(1), dim3(256), 0, st, B->nsys, B->d_tile_first, B->active, B->tile_list,
                        B->tile_count);
@@ -705,6 +785,7 @@ int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, dou
     M((void**)&B->tile_list, sizeof(int) * B->ntiles);
     M((void**)&B->tile_count, sizeof(int));
     M((void**)&B->dinv, sizeof(double) * B->nrows);
+    M((void**)&B->confirm, sizeof(int) * nsys);
     if (e == hipSuccess) e = hipHostMalloc((void**)&B->hcount, sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_batch(B);
@@ -815,24 +896,40 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
         hipLaunchKernelGGL((k_bdots<2>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
                            B->active, P, B->part);
         hipLaunchKernelGGL((k_bsys<2, 4>), dim3(sys_blocks(B->nsys)), dim3(256), 0, st, B->nsys, B->d_tile_first,
-                           B->part, B->st, B->active, B->iters, B->nactive, tol2, 0);
+                           B->part, B->st, B->active, B->iters, B->nactive, B->confirm, tol2, 0);
         hipLaunchKernelGGL(k_bcompact, dim3(1), dim3(256), 0, st, B->nsys, B->d_tile_first, B->active, B->tile_list,
                            B->tile_count);
         HPB_LAUNCH_CHECK();
     }
-    // the captured cycle bakes in the tolerance and the preconditioner choice
-    if (B->exec && (B->graph_tol2 != tol2 || B->graph_jacobi != B->jacobi)) {
-        (void)hipGraphExecDestroy(B->exec);
-        (void)hipGraphDestroy(B->graph);
-        B->exec = nullptr;
-        B->graph = nullptr;
+    HPB_HIP(hipMemsetAsync(B->confirm, 0, sizeof(int) * B->nsys, st));
+    // the captured cycles bake in the tolerance and the preconditioner choice
+    if (B->graph_tol2 != tol2 || B->graph_jacobi != B->jacobi) {
+        for (int k = 0; k < 2; ++k) {
+            hipGraphExec_t& e = k ? B->exec2 : B->exec;
+            hipGraph_t& g = k ? B->graph2 : B->graph;
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+            e = nullptr;
+            g = nullptr;
+        }
     }
-    if (use_graph && !B->exec) {
-        HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        record_cycle(B, tol2);
-        hipError_t ce = hipStreamEndCapture(st, &B->graph);
-        if (ce != hipSuccess) return hpb::hip_status(ce, __FILE__, __LINE__, "hipStreamEndCapture");
-        HPB_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    // cycle c (from 1) recomputes the true residual when recalc divides c
+    const int recalc = B->recalc;
+    auto is_recalc = [recalc](int c) { return recalc > 0 && c % recalc == 0; };
+    if (use_graph) {
+        for (int k = 0; k < 2; ++k) {
+            const bool rc = k == 0;
+            if (rc && recalc == 0) continue;        // never used
+            if (!rc && recalc == 1) continue;
+            hipGraphExec_t& e = rc ? B->exec : B->exec2;
+            hipGraph_t& g = rc ? B->graph : B->graph2;
+            if (e) continue;
+            HPB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            record_cycle(B, tol2, rc);
+            hipError_t ce = hipStreamEndCapture(st, &g);
+            if (ce != hipSuccess) return hpb::hip_status(ce, __FILE__, __LINE__, "hipStreamEndCapture");
+            HPB_HIP(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+        }
         B->graph_tol2 = tol2;
         B->graph_jacobi = B->jacobi;
     }
@@ -843,8 +940,9 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     while (cycles < max_cycles) {
         const int k = std::min(check_every, max_cycles - cycles);
         for (int i = 0; i < k; ++i) {
-            if (use_graph) HPB_HIP(hipGraphLaunch(B->exec, st));
-            else record_cycle(B, tol2);
+            const bool rc = is_recalc(cycles + i + 1);
+            if (use_graph) HPB_HIP(hipGraphLaunch(rc ? B->exec : B->exec2, st));
+            else record_cycle(B, tol2, rc);
         }
         HPB_LAUNCH_CHECK();
         cycles += k;
@@ -869,7 +967,7 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     if (int rc = all_active()) return rc;
     hipLaunchKernelGGL((k_bspmv<1, 0>), dim3(G), dim3(BT_THREADS), 0, st, B->d_tiles, B->tile_list, B->tile_count,
                        B->active, B->rp, B->col, B->val, (const double*)v[VX], (const double*)v[VB], v[VR1],
-                       (const double*)nullptr, (const double*)nullptr, B->part);
+                       (const double*)nullptr, (const double*)nullptr, B->part, (const int*)nullptr);
     {
         dot_pairs P{};
         P.a[0] = v[VR1];
@@ -917,6 +1015,14 @@ int hpb_batch_set_jacobi(hpb_batch_t b, int on)
         HPB_HIP(hipStreamSynchronize(st));
     }
     B->jacobi = on ? 1 : 0;
+    return HPB_OK;
+}
+
+int hpb_batch_set_resid_recalc_freq(hpb_batch_t b, int freq)
+{
+    HPB_REQUIRE(b, HPB_ERR_INVALID, "hpb_batch_set_resid_recalc_freq: null batch");
+    HPB_REQUIRE(freq >= 0, HPB_ERR_INVALID, "hpb_batch_set_resid_recalc_freq: freq must be >= 0");
+    ((batch_t*)b)->recalc = freq;
     return HPB_OK;
 }
 

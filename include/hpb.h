@@ -409,8 +409,9 @@ int hpb_axpy_f32_to_f64(hpb_handle_t h, size_t n, double a, const float* x, doub
  * Philox key seed + s * 0x9E3779B97F4A7C15, harmonic face means as in
  * hpb_csr_diffusion2d_f64) held as one block-diagonal CSR matrix, b = A x* with
  * x* uniform in [-1, 1] (rhs_seed).  Solved together by BiCGStab(2) with one
- * captured hipGraph per cycle (bicgstabl.h:151-274 recurrences, L = 2, true
- * residual recomputed every cycle). */
+ * captured hipGraph per cycle (bicgstabl.h:151-274 recurrences, L = 2; the true
+ * residual confirms convergence, and replaces the recursive one as often as
+ * hpb_batch_set_resid_recalc_freq asks). */
 typedef struct hpb_batch_s* hpb_batch_t;
 int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, double sigma, uint64_t seed,
                                  uint64_t rhs_seed, hpb_batch_t* out);
@@ -426,8 +427,16 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
                               int* out_iters, double* out_true_rel_resid, int* out_converged, double* out_seconds,
                               int* out_cycles);
 /* on != 0: Jacobi right preconditioning, A D^-1 (D x) = b with D = diag(A); the
- * residual tested every cycle is still b - A x of the original system. */
+ * residual tested is still b - A x of the original system. */
 int hpb_batch_set_jacobi(hpb_batch_t b, int on);
+/* The true residual b - A x replaces the recursive one in cycles freq, 2 freq, ...
+ * (freq >= 1; 1: every cycle), as resid_recalc_freq does in the reference's
+ * BiCGStab(L) (bicgstabl.h:247-248; 0, the default there and here: never).  In the
+ * other cycles the convergence test uses the recursive residual of the MR update,
+ * and a system that meets the tolerance gets its true residual computed (replacing
+ * the recursive one) and tested in the same cycle: a system is only ever declared
+ * converged on its true residual.  Saves one of the five SpMVs of such cycles. */
+int hpb_batch_set_resid_recalc_freq(hpb_batch_t b, int freq);
 /* verbose != 0: print the count of active systems to stderr at every check
  * (the role of the reference monitor's log output, default_monitor.h:204-214) */
 int hpb_batch_set_verbose(hpb_batch_t b, int verbose);
