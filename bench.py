@@ -396,7 +396,7 @@ def _run_solve(H, h, S, A, b, xs, p, ir=False, keep_x=False):
         sweep = _spmv_bytes(n, nnz, 8, 8, True) + 8 * n + 12 * n + 20 * n
         out["bytes_moved"] = int(inner + (r["outer_iters"] + 1) * sweep + 12 * nnz)
     else:
-        out["bytes_moved"] = int(_solve_bytes(p, n, nnz, r["iters"]))
+        out["bytes_moved"] = int(_solve_bytes(p, n, nnz, r["iters"], recalc=p.resid_recalc_freq != 0))
     return out
 
 
@@ -433,10 +433,17 @@ def _mixed_on(H, h, S, log, nx, coef, seed, plain, mixed):
     for k, p in mixed.items():
         log(k)
         res[k] = _run_solve(H, h, S, A, b, xs, p, ir=True, keep_x=True)
-    for a, m in zip(plain, mixed):
-        # the mixed-precision solution against the FP64 solve of the same solver
-        res[m]["rel_diff_vs_fp64_solve"] = _rel_diff(H, h, A.rows, res[m]["_x"], res[a]["_x"])
-        if res[a]["converged"] and res[m]["converged"]:
+    for m in mixed:
+        # baselines: the FP64 solves of the same solver that reach 1e-12 true residual;
+        # the speedup is priced against the fastest of them
+        fam = m.split("_")[0]
+        cands = [a for a in plain if a.split("_")[0] == fam]
+        # the mixed-precision solution against the first FP64 solve of the same solver
+        res[m]["rel_diff_vs_fp64_solve"] = _rel_diff(H, h, A.rows, res[m]["_x"], res[cands[0]]["_x"])
+        ok = [a for a in cands if res[a]["converged"] and res[a]["true_rel_resid"] <= 1e-12]
+        if ok and res[m]["converged"] and res[m]["true_rel_resid"] <= 1e-12:
+            a = min(ok, key=lambda k: res[k]["seconds"])
+            res["fp64_baseline_" + m] = a
             res["speedup_" + m] = round(res[a]["seconds"] / res[m]["seconds"], 3)
             res["bytes_ratio_" + m] = round(res[a]["bytes_moved"] / res[m]["bytes_moved"], 3)
     for k in list(plain) + list(mixed):
@@ -449,6 +456,9 @@ def _mixed_on(H, h, S, log, nx, coef, seed, plain, mixed):
 def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
     """FP32-storage inner GMRES / BiCGStab(2) with compensated FP32 dots inside FP64
     refinement vs the plain FP64 compensated solvers, time to 1e-12 true residual.
+    FP64 BiCGStab(2) runs twice: true residual every cycle (the library default) and
+    never (the reference's default resid_recalc_freq 0); each speedup is priced
+    against the fastest FP64 solve of the same solver that reaches 1e-12 true residual.
     GMRES: restarted refinement, one FP32 GMRES solve per sweep to the inner
     tolerance.  BiCGStab(2): reliable updates, one FP32 iteration that is never
     restarted, the FP64 solution and true residual updated each time the FP32
@@ -462,7 +472,8 @@ def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
         res["convdiff128"] = _mixed_on(
             H, h, S, log, 128, S.CONVDIFF7, 2,
             {"gmres30_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=30, **G),
-             "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
+             "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B),
+             "bicgstab2_fp64_recalc0": S.params(rel_tol=1e-12, max_iters=20000, resid_recalc_freq=0, **B)},
             {"gmres30_fp32_refined": S.params(rel_tol=1e-4, max_iters=5000, restart=30, **G),
              "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=20000, refinement=1, **B)})
         res["convdiff128"]["matrix"] = "128^3 convection-diffusion (as krylov)"
@@ -471,7 +482,8 @@ def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
         res["aniso256"] = _mixed_on(
             H, h, S, log, 256, S.ANISO7, 2,
             {"gmres30_fp64": S.params(rel_tol=1e-12, max_iters=20000, restart=30, **G),
-             "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B)},
+             "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B),
+             "bicgstab2_fp64_recalc0": S.params(rel_tol=1e-12, max_iters=20000, resid_recalc_freq=0, **B)},
             {"gmres30_fp32_refined": S.params(rel_tol=1e-2, max_iters=5000, restart=30, compensated_spmv=1, **G),
              "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, **B)})
         res["aniso256"]["matrix"] = "256^3 7-point anisotropic diffusion, k_z / k_x = 1e4, Dirichlet"
