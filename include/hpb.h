@@ -354,7 +354,10 @@ typedef struct hpb_solver_params {
                                 (Daniel-Gragg-Kaufman-Stewart selective reorthogonalisation) */
     int early_exit;          /* GMRES: leave a cycle when the Givens estimate meets the tolerance */
     int reorthogonalization; /* GMRES MGS: one extra correction per basis vector */
-    int resid_recalc_freq;   /* BiCGStab(L) / BiCGStab: recompute b - Ax every k cycles / steps (0 never) */
+    int resid_recalc_freq;   /* BiCGStab(L) / BiCGStab: recompute b - Ax every k cycles / steps (0 never;
+                                the library default is 1, the reference's 0).  With 0 the monitor, and so
+                                `converged`, sees the recursive residual, as in the reference; the
+                                result's true_rel_resid is the check on b - Ax. */
     int save_history;        /* keep (iteration, ||r||/||b||) pairs */
     int verbose;             /* 0 silent, 1 errors, 2 per-check info */
     int device_resident;     /* 1: iterations run on the device (scalars in device memory, one captured
