@@ -392,7 +392,8 @@ def _run_solve(H, h, S, A, b, xs, p, ir=False, keep_x=False):
         # inner FP32 solves (FP32 values, int32 indices, FP32 vectors) + per sweep:
         # FP64 compensated residual, its norm, scaling to FP32, FP64 update; plus the
         # FP32 copy of the values
-        inner = _solve_bytes(p, n, nnz, r["iters"], sval=4, s=4, recalc=p.refinement != 1)
+        inner = _solve_bytes(p, n, nnz, r["iters"], sval=4, s=4,
+                             recalc=p.refinement != 1 and p.resid_recalc_freq != 0)
         sweep = _spmv_bytes(n, nnz, 8, 8, True) + 8 * n + 12 * n + 20 * n
         out["bytes_moved"] = int(inner + (r["outer_iters"] + 1) * sweep + 12 * nnz)
     else:
@@ -460,9 +461,10 @@ def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
     never (the reference's default resid_recalc_freq 0); each speedup is priced
     against the fastest FP64 solve of the same solver that reaches 1e-12 true residual.
     GMRES: restarted refinement, one FP32 GMRES solve per sweep to the inner
-    tolerance.  BiCGStab(2): reliable updates, one FP32 iteration that is never
-    restarted, the FP64 solution and true residual updated each time the FP32
-    residual has dropped 10x."""
+    tolerance.  BiCGStab(2) two ways: reliable updates, one FP32 iteration that is
+    never restarted, the FP64 solution and true residual updated each time the FP32
+    residual has dropped 10x; and restarted refinement, one FP32 solve per sweep to
+    the inner tolerance 1e-2."""
     import high_precision_blas1_amd.solvers as S
     res = {}
     G = dict(solver=S.GMRES, high_precision=1)
@@ -475,7 +477,9 @@ def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
              "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B),
              "bicgstab2_fp64_recalc0": S.params(rel_tol=1e-12, max_iters=20000, resid_recalc_freq=0, **B)},
             {"gmres30_fp32_refined": S.params(rel_tol=1e-4, max_iters=5000, restart=30, **G),
-             "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=20000, refinement=1, **B)})
+             "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=20000, refinement=1, **B),
+             "bicgstab2_fp32_refined": S.params(rel_tol=1e-2, max_iters=20000, refinement=0, resid_recalc_freq=0,
+                                                **B)})
         res["convdiff128"]["matrix"] = "128^3 convection-diffusion (as krylov)"
     if "aniso256" in problems:
         log("mixed precision 256^3 anisotropic diffusion")
@@ -485,7 +489,9 @@ def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
              "bicgstab2_fp64": S.params(rel_tol=1e-12, max_iters=20000, **B),
              "bicgstab2_fp64_recalc0": S.params(rel_tol=1e-12, max_iters=20000, resid_recalc_freq=0, **B)},
             {"gmres30_fp32_refined": S.params(rel_tol=1e-2, max_iters=5000, restart=30, compensated_spmv=1, **G),
-             "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, **B)})
+             "bicgstab2_fp32_reliable": S.params(rel_tol=0.1, max_iters=40000, refinement=1, **B),
+             "bicgstab2_fp32_refined": S.params(rel_tol=1e-2, max_iters=40000, refinement=0, resid_recalc_freq=0,
+                                                **B)})
         res["aniso256"]["matrix"] = "256^3 7-point anisotropic diffusion, k_z / k_x = 1e4, Dirichlet"
     return res
 
