@@ -76,12 +76,13 @@ static void spmv_launch_l(hpb_handle_t h, int grid, int rows, const int* rp, con
 }
 
 // About two entries per lane: on 7-point stencils (7 per row) 4 lanes beat 8 by
-// 15 % at 256^3 (4067 vs 3536 GB/s) and 2 lanes by 19 %.
+// 15 % at 256^3 (4067 vs 3536 GB/s) and 2 lanes by 19 %; on 5-point stencils (5 per
+// row) 2 lanes beat 4 by 13 % at 4096^2 (4052 vs 3529 GB/s) and 1 lane by 26 %.
 static int pick_lpr(long rows, long nnz)
 {
     const double mean = rows > 0 ? (double)nnz / (double)rows : 1.0;
     if (mean <= 1.5) return 1;
-    if (mean <= 3.0) return 2;
+    if (mean <= 6.0) return 2;
     if (mean <= 12.0) return 4;
     if (mean <= 24.0) return 8;
     if (mean <= 48.0) return 16;

@@ -1358,7 +1358,7 @@ private:
     {
         hpb_handle_t h = P.h;
         const double mean = P.rows > 0 ? (double)P.nnz / P.rows : 1.0;
-        const int lpr = mean <= 12.0 ? 4 : (mean <= 24.0 ? 8 : 16);   // as the library SpMV
+        const int lpr = mean <= 6.0 ? 2 : (mean <= 12.0 ? 4 : (mean <= 24.0 ? 8 : 16));   // as the library SpMV
         const long per_block = DB / lpr;
         long grid = (P.rows + per_block - 1) / per_block;
         const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;   // the gathers want many waves
@@ -1373,7 +1373,8 @@ private:
         if (cr) { if (comp) DS_SPMV(L_, true, true); else DS_SPMV(L_, true, false); }                        \
         else { if (comp) DS_SPMV(L_, false, true); else DS_SPMV(L_, false, false); }                         \
     } while (0)
-        if (lpr == 4) DS_SPMV_L(4);
+        if (lpr == 2) DS_SPMV_L(2);
+        else if (lpr == 4) DS_SPMV_L(4);
         else if (lpr == 8) DS_SPMV_L(8);
         else DS_SPMV_L(16);
 #undef DS_SPMV_L
