@@ -36,14 +36,16 @@ const char* hpb_version(void) { return "hpb 0.1 gfx950"; }
 static void default_tuning(hpb_handle_s* h)
 {
     // Shapes measured fastest on MI355X at N = 2^27 (FP64): two-input reductions
-    // 512 threads x 4 16-byte loads per input, 3 blocks per 4 CUs, nontemporal
-    // loads; one-input reductions the same with one block per CU; elementwise
+    // 512 threads x 2 16-byte loads per input, 3 blocks per 4 CUs, nontemporal
+    // loads (7088 GB/s against 6979 with 4 loads, interleaved in one process; the
+    // autotuner's choice on another box as well); one-input reductions 4 loads
+    // and one block per CU; elementwise
     // 256 threads x 1 vector, 2 blocks per CU, nontemporal loads and stores (axpy
     // 6349 GB/s against 5447 GB/s with 2 vectors and cached accesses).  More
     // resident blocks lower the achieved bandwidth of these streams.
     const int cu = h->num_cu > 0 ? h->num_cu : 256;
     for (int d = 0; d < 2; ++d) {
-        h->tune_red[d] = {512, 4, (3 * cu) / 4 > 0 ? (3 * cu) / 4 : 1, HPB_TUNE_NONTEMPORAL};
+        h->tune_red[d] = {512, 2, (3 * cu) / 4 > 0 ? (3 * cu) / 4 : 1, HPB_TUNE_NONTEMPORAL};
         h->tune_red1[d] = {512, 4, cu, HPB_TUNE_NONTEMPORAL};
         h->tune_map[d] = {256, 1, 2 * cu, HPB_TUNE_NONTEMPORAL};
         // multi-dot: 8 columns per chunk, 8 blocks per CU over all chunks, nontemporal
