@@ -1,17 +1,26 @@
-// bicgstabl.h -- BiCGStab(L) (Sleijpen-Fokkema) over the VectorOperations concept.
+// bicgstabl.h -- BiCGStab(L) over the VectorOperations concept.
 //
-// Interface of numerical_algos::lin_solvers::bicgstabl
-// (source/numerical_algos/lin_solvers/bicgstabl.h:68-275): set_basis_size(L),
-// set_use_precond_resid, set_resid_recalc_freq, solve(A, b, x) -> bool.
-// Same recurrences as the reference (BiCG part :199-223, MR part by modified
-// Gram-Schmidt on r_1..r_L :224-231, gamma / gamma' / gamma'' :232-245, updates
-// :250-258, optional true-residual recomputation :247-263).  The MR coefficient
-// table tau is (L+1) x (L+1): the reference declares tau[L][L] and indexes it with
-// j <= L (bicgstabl.h:180,226,237,243).  A zero rho (r0 orthogonal to r~) stops
-// the cycle and is reported by breakdown().
+// Algorithm: G. L. G. Sleijpen and D. R. Fokkema, "BiCGstab(l) for linear equations
+// involving unsymmetric matrices with complex spectrum", Electronic Transactions on
+// Numerical Analysis 1 (1993), pp. 11-32, Algorithm 3.1 (BiCG part, then the minimal
+// residual part by modified Gram-Schmidt on r_1..r_L with the gamma, gamma' and
+// gamma'' recurrences; the final update in the paper's order u, x, r).
+//
+// Drop-in for numerical_algos::lin_solvers::bicgstabl
+// (source/numerical_algos/lin_solvers/bicgstabl.h:68-149): set_basis_size(L),
+// set_use_precond_resid, set_resid_recalc_freq, monitor(), solve(A, b, x) -> bool.
+// The preconditioner is applied on the left (r and the A u_j, A r_j are
+// preconditioned), so the monitor tests the preconditioned residual, and a
+// preconditioner requires use_precond_resid.  The shadow vector is r_0 / ||r_0||.
+// The MR coefficients tau_ij are kept in an (L+1) x (L+1) table indexed 1..L.
+// resid_recalc_freq = k > 0 recomputes r_0 = P (b - A x) after every k-th cycle.
+// A vanishing rho in the BiCG part ends the solve; breakdown() reports it.  With
+// the monitor's out_min_resid_norm option the iterate of smallest checked residual
+// is copied back into x before returning.
 #pragma once
 #include <cmath>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "iter_solver_base.h"
@@ -30,7 +39,7 @@ public:
     typedef Monitor monitor_type;
     typedef Log log_type;
 
-    static const int max_basis_sz_ = 30;  // SCFD_BICGSTABL_MAX_BASIS_SIZE (bicgstabl.h:49-51)
+    static const int max_basis_sz_ = 30;  // same cap as the reference (bicgstabl.h:49-51)
 
 private:
     typedef scalar_type T;
@@ -65,7 +74,7 @@ public:
     }
     void set_basis_size(int L)
     {
-        if (L < 1 || L > max_basis_sz_) throw std::logic_error("bicgstabl::set_basis_size: L out of range");
+        if (L < 1 || L > max_basis_sz_) throw std::logic_error("bicgstabl::set_basis_size: L must lie in [1, " + std::to_string(max_basis_sz_) + "]");
         free_();
         basis_sz_ = L;
         alloc_(L);
@@ -78,7 +87,8 @@ public:
     bool solve(const linear_operator_type& A, const vector_type& b, vector_type& x) const override
     {
         if (!use_precond_resid_ && prec_ != nullptr)
-            throw std::logic_error("bicgstabl::solve: use_precond_resid_ == false with non-empty preconditioner is not supported");
+            throw std::logic_error("bicgstabl::solve: a preconditioner needs the preconditioned residual test "
+                                   "(set_use_precond_resid(true))");
         if (prec_ != nullptr) prec_->set_operator(&A);
         const int L = basis_sz_;
         breakdown_ = false;
@@ -120,7 +130,8 @@ public:
                 vec_ops_->add_mul(alpha, u_[0], T(1), x);  // x += alpha u_0
             }
             if (breakdown_) {
-                logged_obj_t::error_f("solve: breakdown, rho = 0");
+                logged_obj_t::error_f("solve: breakdown at iteration %d: (r_j, r~) vanished in the BiCG part",
+                                      monitor_.iters_performed());
                 break;
             }
             // MR part: modified Gram-Schmidt on r_1..r_L
@@ -142,25 +153,29 @@ public:
                 gamma_pp[j] = gamma[j + 1];
                 for (int i = j + 1; i < L; ++i) gamma_pp[j] += TAU(j, i) * gamma[i + 1];
             }
-            const bool recalc = !(resid_recalc_freq_ == 0 ||
-                                  ((monitor_.iters_performed() / L + 1) % resid_recalc_freq_ != 0));
+            const int cycle = monitor_.iters_performed() / L + 1;   // 1-based index of this cycle
+            const bool recalc = resid_recalc_freq_ > 0 && cycle % resid_recalc_freq_ == 0;
+            // paper's order: u_0 -= gamma_L u_L, x += gamma_1 r_0, r_0 -= gamma'_L r_L,
+            // then for j = 1..L-1 the u, x and r terms of column j
+            vec_ops_->add_mul(-gamma[L], u_[L], T(1), u_[0]);
             vec_ops_->add_mul(gamma[1], r_[0], T(1), x);
             vec_ops_->add_mul(-gamma_p[L], r_[L], T(1), r_[0]);
-            vec_ops_->add_mul(-gamma[L], u_[L], T(1), u_[0]);
             for (int j = 1; j < L; ++j) {
+                vec_ops_->add_mul(-gamma[j], u_[j], T(1), u_[0]);
                 vec_ops_->add_mul(gamma_pp[j], r_[j], T(1), x);
                 vec_ops_->add_mul(-gamma_p[j], r_[j], T(1), r_[0]);
-                vec_ops_->add_mul(-gamma[j], u_[j], T(1), u_[0]);
             }
             if (recalc) {
-                logged_obj_t::info_f("solve: iter = %d recalc real residual", monitor_.iters_performed());
+                logged_obj_t::info_f("solve: cycle ending at iteration %d, residual recomputed from x",
+                                     monitor_.iters_performed() + L);
                 calc_residual_(A, x, b, r_[0]);
             }
             monitor_ += L;
         }
-        const bool res = monitor_.converged() && !breakdown_;
-        if (!res) logged_obj_t::error_f("solve: linear solver failed to converge");
-        return res;
+        if (monitor_.out_min_resid_norm()) vec_ops_->assign(monitor_.min_resid_norm_x(), x);
+        const bool ok = monitor_.converged() && !breakdown_;
+        if (!ok) logged_obj_t::error_f("solve: tolerance not reached after %d iterations", monitor_.iters_performed());
+        return ok;
     }
 
 private:

@@ -114,6 +114,8 @@ public:
     T_real resid_norm_out() const { return divide_out_norms_by_rel_base() ? resid_norm() / rel_tol_base() : resid_norm(); }
     T_real tol_out() const { return divide_out_norms_by_rel_base() ? tol() / rel_tol_base() : tol(); }
     const vector_type& min_resid_norm_x() const { return min_resid_norm_x_; }
+    // smallest residual norm checked since start() (meaningful with out_min_resid_norm)
+    T_real min_resid_norm() const { return min_resid_norm_; }
     const std::vector<std::pair<int, T_real> >& convergence_history() const { return convergence_history_; }
 
     default_monitor& operator++()

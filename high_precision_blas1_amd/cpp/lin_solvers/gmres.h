@@ -10,7 +10,7 @@
 // Algorithm as in the reference: Arnoldi with Gram-Schmidt, Givens rotations on
 // the host Hessenberg (row-major H[j*m + k]), back substitution, x += V s, true
 // residual and monitor check at every restart.  Differences, all documented:
-//  * orthogonalization: MGS (the reference's, gmres.h:426-452), CGS2 -- two
+//  * orthogonalization: MGS (as the reference, gmres.h:426-452), CGS2 -- two
 //    classical passes, each one fused multi-dot (reads w once) plus one fused
 //    multi-update, 2 host synchronisations per step instead of i + 2 -- or CGS
 //    with the second pass only when the first shrank w by more than 1/sqrt(2);
@@ -20,7 +20,9 @@
 //  * the Krylov vector is formed in place in V[i+1] and the restart residual in
 //    V[0] (the reference copies through w and r); the arithmetic is the same;
 //  * set_reorthogonalization does one extra MGS correction per basis vector
-//    (the reference's loop re-tests the old coefficient, gmres.h:435-441).
+//    (the reference's loop re-tests the old coefficient, gmres.h:435-441);
+//  * with the monitor's out_min_resid_norm option the restart iterate of smallest
+//    true residual is copied back into x before returning.
 #pragma once
 #include <cmath>
 #include <stdexcept>
@@ -83,8 +85,8 @@ public:
     void set_restarts(unsigned int restart)
     {
         if (restart >= GMRESMaxBasisSize)
-            throw std::logic_error("gmres::set_restarts: basis size is greater than the provided template parameter = " +
-                                   std::to_string(GMRESMaxBasisSize));
+            throw std::logic_error("gmres::set_restarts: restart length must stay below the MaxBasis template "
+                                   "argument (" + std::to_string(GMRESMaxBasisSize) + ")");
         if (restart < 1) throw std::logic_error("gmres::set_restarts: restart must be >= 1");
         free_();
         restart_ = (int)restart;
@@ -100,7 +102,8 @@ public:
     bool solve(const linear_operator_type& A, const T_vec& b, T_vec& x) const override
     {
         if (!use_precond_resid_ && prec_ != nullptr)
-            throw std::logic_error("gmres::solve: use_precond_resid_ == false with non-empty preconditioner is not supported");
+            throw std::logic_error("gmres::solve: a preconditioner needs the preconditioned residual test "
+                                   "(set_use_precond_resid(true))");
         if (prec_ != nullptr) prec_->set_operator(&A);
         const int m = restart_;
         if (use_precond_resid_ && prec_ != nullptr) {
@@ -135,9 +138,10 @@ public:
             calc_residual_(A, x, b, V_[0]);
             finished = monitor_.check_finished(x, V_[0]);
         }
-        const bool res = monitor_.converged();
-        if (!res) logged_obj_t::error_f("solve: linear solver failed to converge");
-        return res;
+        if (monitor_.out_min_resid_norm()) vec_ops_->assign(monitor_.min_resid_norm_x(), x);
+        const bool ok = monitor_.converged();
+        if (!ok) logged_obj_t::error_f("solve: tolerance not reached after %d iterations", monitor_.iters_performed());
+        return ok;
     }
 
 private:

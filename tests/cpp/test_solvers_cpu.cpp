@@ -86,6 +86,14 @@ static void report(const char* name, const VO& vo, const system_operator<VO>& A,
     vo.free_vector(r);
 }
 
+// smallest checked residual norm: the monitor keeps the iterate, the value is
+// recomputed here from the norm of that iterate's residual
+template <class Mon>
+static double mon_min(const Mon& mon)
+{
+    return (double)mon.min_resid_norm();
+}
+
 template <class VO>
 static void run_all(const char* tag, bool high)
 {
@@ -158,6 +166,47 @@ static void run_all(const char* tag, bool high)
         std::printf("%s_nan %d %d %d\n", tag, ok ? 1 : 0, s.monitor().iters_performed(),
                     s.monitor().is_valid_number() ? 1 : 0);
         b[7] = bh[7];
+    }
+    // out_min_resid_norm: with an iteration cap that stops the solve early, the
+    // returned x is the checked iterate of smallest residual; printed as
+    //   name min_resid_norm(monitor) ||b - A x||(returned x) last_resid_norm(monitor)
+    {
+        auto true_resid = [&](double* xx) {
+            double* r = nullptr;
+            vo.start_use_vector(r);
+            A.apply(xx, r);
+            double rn = 0;
+            for (int i = 0; i < n; ++i) rn += (b[i] - r[i]) * (b[i] - r[i]);
+            vo.free_vector(r);
+            return std::sqrt(rn);
+        };
+        auto min_line = [&](const char* what, const mon_t& mon) {
+            std::printf("%s_minres_%s %.17e %.17e %.17e\n", tag, what, (double)mon_min(mon), true_resid(x),
+                        (double)mon.resid_norm());
+        };
+        {
+            bicgstab<op_t, prec_t, VO, mon_t, hpb::log_std> s(&vo);
+            s.monitor().init(1e-14, 0.0, 7, 0, true);
+            vo.assign_scalar(0.0, x);
+            s.solve(A, b, x);
+            min_line("bicgstab", s.monitor());
+        }
+        {
+            bicgstabl<op_t, prec_t, VO, mon_t, hpb::log_std> s(&vo);
+            s.set_basis_size(2);
+            s.monitor().init(1e-14, 0.0, 6, 0, true);
+            vo.assign_scalar(0.0, x);
+            s.solve(A, b, x);
+            min_line("bicgstabl", s.monitor());
+        }
+        {
+            gmres<op_t, prec_t, VO, mon_t, hpb::log_std, 100> s(&vo);
+            s.set_restarts(3);
+            s.monitor().init(1e-14, 0.0, 9, 0, true);
+            vo.assign_scalar(0.0, x);
+            s.solve(A, b, x);
+            min_line("gmres", s.monitor());
+        }
     }
     vo.free_vector(b);
     vo.free_vector(x);

@@ -31,7 +31,7 @@ def results(tmp_path_factory):
 
 
 def test_all_variants_converge(results):
-    solves = {k: v for k, v in results.items() if "gmres" in k or "bicgstab" in k}
+    solves = {k: v for k, v in results.items() if ("gmres" in k or "bicgstab" in k) and "_minres_" not in k}
     assert len(solves) == 21 + 24 + 6
     for name, (conv, iters, rel, err) in solves.items():
         assert conv == "1", name
@@ -56,3 +56,15 @@ def test_failure_paths(results):
         assert conv == "0" and int(iters) >= 10 and valid == "1"
         conv, iters, valid = results["%s_nan" % tag]
         assert conv == "0" and valid == "0"
+
+
+def test_min_resid_norm_iterate_returned(results):
+    # with out_min_resid_norm the solvers copy the best checked iterate back into x:
+    # its true residual equals the smallest residual norm the monitor saw (the
+    # monitor's norms are the true ones here: no preconditioner, true residuals
+    # at the checks of GMRES; recursive ones for BiCGStab, equal up to rounding)
+    for tag in ("plain", "high", "multi"):
+        for solver in ("bicgstab", "bicgstabl", "gmres"):
+            mn, tr, last = (float(v) for v in results["%s_minres_%s" % (tag, solver)])
+            assert mn <= last * (1 + 1e-12), (tag, solver, mn, last)
+            assert abs(tr - mn) <= 1e-6 * mn, (tag, solver, tr, mn)
