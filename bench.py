@@ -145,11 +145,17 @@ def cpu_baseline(H, h, x, y, n, budget_s=12.0):
 
     model = "unknown"
     try:
+        info = {}
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    info.setdefault(k.strip(), v.strip())
+                if not line.strip() and info:
                     break
+        model = "%s (vendor %s, family %s, model %s, stepping %s, %s threads)" % (
+            info.get("model name", "?"), info.get("vendor_id", "?"), info.get("cpu family", "?"),
+            info.get("model", "?"), info.get("stepping", "?"), os.cpu_count())
     except OSError:
         pass
     t_thr, reps_thr, tmin, tmax = run(fthr, budget_s * 0.6)

@@ -65,6 +65,28 @@ def build(verbose=True, jobs=None):
     return LIB
 
 
+TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+TEST_BIN = os.path.join(ROOT, "tests", "cpp", "bin", "test_dropin")
+
+
+def build_tests(verbose=True):
+    """The drop-in C++ test program (tests/cpp/test_dropin.cpp), host code linked
+    against libhpb.so; run by tests/test_gpu_dropin.py on the GPU."""
+    os.makedirs(os.path.dirname(TEST_BIN), exist_ok=True)
+    tmp = TEST_BIN + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-Werror", "-ffp-contract=off",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "cpp"),
+           TEST_SRC, "-o", tmp, "-L", PKG, "-lhpb", "-Wl,-rpath,$ORIGIN/../../../high_precision_blas1_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("test program build failed:\n%s%s" % (r.stdout, r.stderr))
+    os.replace(tmp, TEST_BIN)
+    return TEST_BIN
+
+
 if __name__ == "__main__":
     build()
+    build_tests()
     sys.exit(0)

@@ -7,8 +7,9 @@
 // Zero-based int32 ordinals like the reference's default.  Unlike the reference
 // there is no per-call allocation; rows can be evaluated as compensated dot
 // products (use_compensated_rows), and a fused residual r := b - A x is offered.
-// Transposed products are not supported (no solver in the path uses them) and
-// throw std::logic_error.
+// Transposed products (axpy and the triangular solves with transpose = true) run
+// the ordinary kernels on A^T, formed once on the device (hpb_csr_transpose_*) on
+// the first transposed call and kept until the matrix changes.
 // triangular_solve_lower / _upper (gpu_matrix.h:142-320) solve with the lower
 // triangle and a unit diagonal / the upper triangle and the stored diagonal; the
 // level analysis of each triangle is made on first use and kept until the
@@ -97,21 +98,27 @@ public:
     bool compensated_rows() const { return comp_; }
 
     // y := alpha*A*x + beta*y
+    // y := alpha*A^T*x + beta*y with transpose (x of length rows, y of length columns)
     void axpy(const T& alpha, const vector_type& x, const T& beta, vector_type& y, bool transpose = false) const
     {
-        if (transpose) throw std::logic_error("csr::hip_matrix::axpy: transposed product not supported");
         auto fn = comp_ ? hpb::abi<T>::csr_spmvc : hpb::abi<T>::csr_spmv;
+        if (transpose) {
+            transpose_();
+            hpb::check(fn(h_, dim_.columns, dim_.rows, nnz_, t_row_ptr_, t_col_ind_, t_data_, alpha, x, beta, y),
+                       "csr::hip_matrix::axpy (transposed)");
+            return;
+        }
         hpb::check(fn(h_, dim_.rows, dim_.columns, nnz_, row_ptr_, col_ind_, data_, alpha, x, beta, y),
                    "csr::hip_matrix::axpy");
     }
 
-    // L x = alpha b: strictly lower triangle, unit diagonal
+    // L x = alpha b: strictly lower triangle, unit diagonal (transpose: L^T x = alpha b)
     void triangular_solve_lower(const vector_type& b, vector_type& x, const T& alpha = T(1.0),
                                 bool transpose = false) const
     {
         trsv_(0, b, x, alpha, transpose);
     }
-    // U x = alpha b: upper triangle with the stored diagonal
+    // U x = alpha b: upper triangle with the stored diagonal (transpose: U^T x = alpha b)
     void triangular_solve_upper(const vector_type& b, vector_type& x, const T& alpha = T(1.0),
                                 bool transpose = false) const
     {
@@ -120,6 +127,7 @@ public:
     // in-place ILU(0) of the values (pattern kept): L strictly below, U on and above
     void ilu0_in_place()
     {
+        drop_transpose_();
         hpb::check(hpb::abi<T>::csr_ilu0(h_, analysis_(0), data_), "csr::hip_matrix::ilu0");
     }
 
@@ -139,17 +147,56 @@ private:
                        "csr::hip_matrix: triangular analysis");
         return a;
     }
+    // on A^T: L^T is the strictly upper triangle of A^T with a unit diagonal (kind 3),
+    // U^T the lower triangle of A^T with the stored diagonal (kind 2)
+    hpb_trsv_t analysis_t_(int uplo) const
+    {
+        transpose_();
+        hpb_trsv_t& a = uplo == 0 ? t_lower_ : t_upper_;
+        if (!a)
+            hpb::check(hpb_csr_trsv_analysis(h_, uplo == 0 ? 3 : 2, dim_.columns, nnz_, t_row_ptr_, t_col_ind_, &a),
+                       "csr::hip_matrix: transposed triangular analysis");
+        return a;
+    }
     void trsv_(int uplo, const vector_type& b, vector_type& x, const T& alpha, bool transpose) const
     {
-        if (transpose) throw std::logic_error("csr::hip_matrix: transposed triangular solve not supported");
+        if (transpose) {
+            hpb::check(hpb::abi<T>::csr_trsv(h_, analysis_t_(uplo), t_data_, alpha, b, x, comp_ ? 1 : 0),
+                       "csr::hip_matrix: transposed triangular solve");
+            return;
+        }
         hpb::check(hpb::abi<T>::csr_trsv(h_, analysis_(uplo), data_, alpha, b, x, comp_ ? 1 : 0),
                    "csr::hip_matrix: triangular solve");
+    }
+    void transpose_() const
+    {
+        if (t_row_ptr_) return;
+        hpb::check(hpb_malloc((void**)&t_row_ptr_, sizeof(int) * ((size_t)dim_.columns + 1)), "csr::hip_matrix: A^T");
+        hpb::check(hpb_malloc((void**)&t_col_ind_, sizeof(int) * (size_t)nnz_), "csr::hip_matrix: A^T");
+        hpb::check(hpb_malloc((void**)&t_data_, sizeof(T) * (size_t)nnz_), "csr::hip_matrix: A^T");
+        hpb::check(hpb::abi<T>::csr_transpose(h_, dim_.rows, dim_.columns, nnz_, row_ptr_, col_ind_, data_, t_row_ptr_,
+                                              t_col_ind_, t_data_),
+                   "csr::hip_matrix: A^T");
     }
     void drop_analyses_()
     {
         hpb_trsv_destroy(lower_);
         hpb_trsv_destroy(upper_);
         lower_ = upper_ = nullptr;
+        drop_transpose_();
+    }
+    // A^T holds a copy of the values: dropped whenever they change
+    void drop_transpose_() const
+    {
+        hpb_trsv_destroy(t_lower_);
+        hpb_trsv_destroy(t_upper_);
+        t_lower_ = t_upper_ = nullptr;
+        if (t_row_ptr_) hpb_free(t_row_ptr_);
+        if (t_col_ind_) hpb_free(t_col_ind_);
+        if (t_data_) hpb_free(t_data_);
+        t_row_ptr_ = nullptr;
+        t_col_ind_ = nullptr;
+        t_data_ = nullptr;
     }
     void release_()
     {
@@ -173,7 +220,10 @@ private:
     bool initialized_ = false;
     bool comp_ = false;
     bool own_ = true;
-    mutable hpb_trsv_t lower_ = nullptr, upper_ = nullptr;
+    mutable hpb_trsv_t lower_ = nullptr, upper_ = nullptr, t_lower_ = nullptr, t_upper_ = nullptr;
+    mutable int* t_row_ptr_ = nullptr;   // A^T, made on the first transposed call
+    mutable int* t_col_ind_ = nullptr;
+    mutable T* t_data_ = nullptr;
 };
 
 }  // namespace csr

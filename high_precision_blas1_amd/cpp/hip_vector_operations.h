@@ -44,7 +44,8 @@ template <> struct abi<double> {
     HPB_F(axpbypcwpdz) HPB_F(assign_mul4) HPB_F(mul_pointwise) HPB_F(mul_pointwise_inplace) HPB_F(div_pointwise)
     HPB_F(div_pointwise_inplace) HPB_F(axpbyc) HPB_F(valid) HPB_F(set_value) HPB_F(get_value)
     HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc) HPB_F(multidot) HPB_F(multiaxpy) HPB_FA(multidot)
-    HPB_F(multiaxpy_dev) HPB_F(csr_trsv) HPB_F(csr_ilu0)
+    HPB_F(multiaxpy_dev) HPB_F(csr_trsv) HPB_F(csr_ilu0) HPB_F(minmax) HPB_F(exact_dot)
+    HPB_F(csr_transpose)
 #undef HPB_F
 #undef HPB_FA
 };
@@ -57,7 +58,8 @@ template <> struct abi<float> {
     HPB_F(axpbypcwpdz) HPB_F(assign_mul4) HPB_F(mul_pointwise) HPB_F(mul_pointwise_inplace) HPB_F(div_pointwise)
     HPB_F(div_pointwise_inplace) HPB_F(axpbyc) HPB_F(valid) HPB_F(set_value) HPB_F(get_value)
     HPB_F(random_uniform) HPB_F(csr_spmv) HPB_F(csr_spmvc) HPB_F(multidot) HPB_F(multiaxpy) HPB_FA(multidot)
-    HPB_F(multiaxpy_dev) HPB_F(csr_trsv) HPB_F(csr_ilu0)
+    HPB_F(multiaxpy_dev) HPB_F(csr_trsv) HPB_F(csr_ilu0) HPB_F(minmax) HPB_F(exact_dot)
+    HPB_F(csr_transpose)
 #undef HPB_F
 #undef HPB_FA
 };
@@ -264,7 +266,7 @@ struct hip_vector_operations {
     {
         check(abi<T>::assign_mul4(h_, sz_, mul_x, x, mul_y, y, mul_v, v, mul_w, w, z), "assign_mul");
     }
-    // compensated z := mul_x*x + mul_y*y, one final rounding
+    // compensated z := mul_x*x + mul_y*y (error-free split, faithful result)
     void assign_mul_compensated(const scalar_type mul_x, const vector_type& x, const scalar_type mul_y,
                                 const vector_type& y, vector_type& z) const
     {

@@ -55,7 +55,10 @@ template <typename T> struct f_mulpw { T a, b; __device__ T operator()(T x, T y,
 template <typename T> struct f_mulpw_ip { T b; __device__ T operator()(T x, T y, T, T) const { return x * (b * y); } };
 template <typename T> struct f_divpw { T a, b; __device__ T operator()(T x, T y, T, T) const { return (a * x) / (b * y); } };
 template <typename T> struct f_divpw_ip { T b; __device__ T operator()(T x, T y, T, T) const { return x / (b * y); } };
-// z = a*x + b*y, error-free until the final rounding.
+// z = a*x + b*y: a x + b y = s + t + e1 + e2 exactly (TwoProd, TwoProd, TwoSum);
+// the correction (e1 + e2) + t is rounded twice before the final s + correction,
+// so z is faithful (one of the two doubles around a x + b y), not always the
+// correctly rounded value.
 template <typename T> struct f_axpbyc {
     T a, b;
     __device__ T operator()(T x, T y, T, T) const

@@ -86,6 +86,8 @@ int hpb_create(hpb_handle_t* out, void* stream)
     if ((e = hipHostMalloc(&h->hres, HPB_DRES_BYTES, hipHostMallocDefault)) != hipSuccess)
         return fail(e, __LINE__, "hipHostMalloc(hres)");
     if ((e = hipMalloc((void**)&h->dflag, 64)) != hipSuccess) return fail(e, __LINE__, "hipMalloc(flag)");
+    if ((e = hipMalloc(&h->nrmfix, sizeof(hpb::nrm_fix))) != hipSuccess) return fail(e, __LINE__, "hipMalloc(nrmfix)");
+    if ((e = hipMemset(h->nrmfix, 0, sizeof(hpb::nrm_fix))) != hipSuccess) return fail(e, __LINE__, "hipMemset(nrmfix)");
     for (int i = 0; i < 2; ++i)
         if ((e = hipEventCreate(&h->ev[i])) != hipSuccess) return fail(e, __LINE__, "hipEventCreate");
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, __LINE__, "hipDeviceSynchronize");
@@ -103,6 +105,7 @@ int hpb_destroy(hpb_handle_t h)
     if (h->dres) (void)hipFree(h->dres);
     if (h->hres) (void)hipHostFree(h->hres);
     if (h->dflag) (void)hipFree(h->dflag);
+    if (h->nrmfix) (void)hipFree(h->nrmfix);
     for (int i = 0; i < 2; ++i)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);

@@ -7,15 +7,19 @@
 // Ogita-Rump-Oishi's Alg. 6.1: a random head whose exponents are uniform in
 // [0, b/2], b = log2(cond), and a tail with exponents descending b/2 -> 0 that
 // cancels the running dot.  Entries are multiples of 2^-53 so that the running dot
-// is exact in a 256-bit fixed-point accumulator; the tail is built in chunks of
-// 256 whose last element cancels the EXACT running sum (a short sequential pass
-// over chunks), every other tail element cancels its predecessor's target.
-// The arithmetic is specified so that the host restatement in the oracle gives
-// the same bits (see hpb_oracle.c, "GenDot-style ... generator").
+// is exact in a 256-bit fixed-point accumulator; the tail is built in chunks whose
+// last element cancels the EXACT running sum (a sequential pass over chunks),
+// every other tail element cancels its predecessor's target.  Chunks hold 256
+// elements, doubled until there are at most 4096 of them, so the sequential pass
+// stays short at any length.  The arithmetic is specified so that the host
+// restatement in the oracle gives the same bits (see hpb_oracle.c, "GenDot-style
+// ... generator").
 //
-// Passes: k_head (parallel, per-block exact partials) -> k_head_reduce (one block)
-// -> k_tail_chunks (one thread per chunk, exact chunk partials) -> k_tail_fix
-// (one wave; the chain over chunks is sequential) -> results.
+// Passes: k_head (parallel, per-block exact partials) -> k_reduce_parts (one block)
+// -> k_tail_chunks (one block per chunk: every element but the chunk's last only
+// needs its own and its predecessor's draws, so the lanes take them in parallel
+// and sum their exact products in LDS) -> k_tail_fix (one wave; the chain over
+// chunks is sequential) -> results.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -27,8 +31,17 @@
 
 namespace hpb {
 
-constexpr int GD_CHUNK = 256;
+constexpr int GD_CHUNK_MIN = 256;
+constexpr uint64_t GD_MAX_CHUNKS = 4096;
 constexpr int GD_BLOCK = 256;
+
+// tail chunk length: 256, doubled until the tail has at most GD_MAX_CHUNKS chunks
+static uint64_t gd_chunk_len(uint64_t n_tail)
+{
+    uint64_t c = GD_CHUNK_MIN;
+    while ((n_tail + c - 1) / c > GD_MAX_CHUNKS) c *= 2;
+    return c;
+}
 
 struct gd_draws {
     uint64_t ka, kb, kc;
@@ -158,32 +171,23 @@ __global__ __launch_bounds__(GD_BLOCK) void k_reduce_parts(int nparts, const uin
         }
 }
 
-// One thread per tail chunk.  head_acc = exact head sum (S limbs 0..3).
-__global__ __launch_bounds__(GD_BLOCK) void k_tail_chunks(uint64_t n_head, uint64_t n_tail, double hb, uint64_t seed,
-                                                          const uint64_t* __restrict__ head_acc, double* __restrict__ x,
-                                                          double* __restrict__ y, uint64_t* __restrict__ cpart,
-                                                          double* __restrict__ r_last)
+// One block per tail chunk.  head_acc = exact head sum (S limbs 0..3).  Element j
+// of the chunk (not its last) takes y = grid53((r_j - r_{j-1}) / x_j) from its own
+// draws and its predecessor's (the head sum, rounded, for the first tail element);
+// the chunk's exact partial sums are taken over those elements.
+__global__ __launch_bounds__(GD_BLOCK) void k_tail_chunks(uint64_t n_head, uint64_t n_tail, uint64_t chunk, double hb,
+                                                          uint64_t seed, const uint64_t* __restrict__ head_acc,
+                                                          double* __restrict__ x, double* __restrict__ y,
+                                                          uint64_t* __restrict__ cpart, double* __restrict__ r_last)
 {
-    const uint64_t nchunks = (n_tail + GD_CHUNK - 1) / GD_CHUNK;
-    const uint64_t ch = (uint64_t)blockIdx.x * GD_BLOCK + threadIdx.x;
-    if (ch >= nchunks) return;
-    const uint64_t j0 = ch * GD_CHUNK;
-    uint64_t j1 = j0 + GD_CHUNK;
+    const uint64_t ch = blockIdx.x;
+    const uint64_t j0 = ch * chunk;
+    uint64_t j1 = j0 + chunk;
     if (j1 > n_tail) j1 = n_tail;
-    double r_prev;
-    if (ch == 0) {
-        acc256 Sh;
-        for (int k = 0; k < 4; ++k) Sh.w[k] = head_acc[k];
-        r_prev = acc_round(Sh);
-    } else {
-        const uint64_t jp = j0 - 1;
-        gd_draws d = gd_draw(seed, n_head + jp);
-        r_prev = ldexp(gd_mant(d.kc), (int)gd_exp_tail(hb, jp, n_tail));
-    }
     acc256 S, A;
     acc_zero(S);
     acc_zero(A);
-    for (uint64_t j = j0; j < j1; ++j) {
+    for (uint64_t j = j0 + threadIdx.x; j < j1; j += GD_BLOCK) {
         const uint64_t i = n_head + j;
         gd_draws d = gd_draw(seed, i);
         const int e = (int)gd_exp_tail(hb, j, n_tail);
@@ -191,18 +195,28 @@ __global__ __launch_bounds__(GD_BLOCK) void k_tail_chunks(uint64_t n_head, uint6
         const double r = ldexp(gd_mant(d.kc), e);
         x[i] = xi;
         if (j + 1 < j1) {
+            double r_prev;
+            if (j == 0) {
+                acc256 Sh;
+                for (int k = 0; k < 4; ++k) Sh.w[k] = head_acc[k];
+                r_prev = acc_round(Sh);
+            } else {
+                gd_draws dp = gd_draw(seed, i - 1);
+                r_prev = ldexp(gd_mant(dp.kc), (int)gd_exp_tail(hb, j - 1, n_tail));
+            }
             const double yi = gd_grid53((r - r_prev) / xi);
             y[i] = yi;
             gd_add_prod(S, A, xi, yi);
         } else {
             r_last[ch] = r;
         }
-        r_prev = r;
     }
-    for (int k = 0; k < 4; ++k) {
-        cpart[ch * 8 + k] = S.w[k];
-        cpart[ch * 8 + 4 + k] = A.w[k];
-    }
+    block_sum_acc(S, A);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) {
+            cpart[ch * 8 + k] = S.w[k];
+            cpart[ch * 8 + 4 + k] = A.w[k];
+        }
 }
 
 // Sequential pass over the chunks: the last element of every chunk cancels the
@@ -212,7 +226,8 @@ __global__ __launch_bounds__(GD_BLOCK) void k_tail_chunks(uint64_t n_head, uint6
 // new y values; the arithmetic is the sequential one.
 constexpr int FIX_BATCH = 64;
 
-__global__ __launch_bounds__(64) void k_tail_fix(uint64_t n_head, uint64_t n_tail, const uint64_t* __restrict__ cpart,
+__global__ __launch_bounds__(64) void k_tail_fix(uint64_t n_head, uint64_t n_tail, uint64_t chunk,
+                                                 const uint64_t* __restrict__ cpart,
                                                  const double* __restrict__ r_last, double* __restrict__ x,
                                                  double* __restrict__ y, uint64_t* __restrict__ acc,
                                                  double* __restrict__ out3)
@@ -224,14 +239,14 @@ __global__ __launch_bounds__(64) void k_tail_fix(uint64_t n_head, uint64_t n_tai
         S.w[k] = acc[k];
         A.w[k] = acc[4 + k];
     }
-    const uint64_t nchunks = (n_tail + GD_CHUNK - 1) / GD_CHUNK;
+    const uint64_t nchunks = (n_tail + chunk - 1) / chunk;
     const int lane = threadIdx.x;
     for (uint64_t c0 = 0; c0 < nchunks; c0 += FIX_BATCH) {
         const int nb = (int)((nchunks - c0) < (uint64_t)FIX_BATCH ? (nchunks - c0) : (uint64_t)FIX_BATCH);
         for (int q = lane; q < nb * 8; q += 64) sp[q] = cpart[c0 * 8 + q];
         if (lane < nb) {
             const uint64_t ch = c0 + lane;
-            uint64_t j1 = (ch + 1) * GD_CHUNK;
+            uint64_t j1 = (ch + 1) * chunk;
             if (j1 > n_tail) j1 = n_tail;
             sx[lane] = x[n_head + j1 - 1];
             sr[lane] = r_last[ch];
@@ -255,7 +270,7 @@ __global__ __launch_bounds__(64) void k_tail_fix(uint64_t n_head, uint64_t n_tai
         __syncthreads();
         if (lane < nb) {
             const uint64_t ch = c0 + lane;
-            uint64_t j1 = (ch + 1) * GD_CHUNK;
+            uint64_t j1 = (ch + 1) * chunk;
             if (j1 > n_tail) j1 = n_tail;
             y[n_head + j1 - 1] = sy[lane];
         }
@@ -287,7 +302,8 @@ extern "C" int hpb_gendot_f64(hpb_handle_t h, size_t n, double log2cond, uint64_
     HPB_REQUIRE(hb >= 0.0 && log2cond + 4.0 + std::log2((double)n) <= 147.0, HPB_ERR_INVALID,
                 "hpb_gendot_f64: condition number out of range (need 1 <= cond, log2(cond) + log2(n) <= 143)");
     const size_t n_head = n - n_tail;
-    const size_t nchunks = (n_tail + GD_CHUNK - 1) / GD_CHUNK;
+    const uint64_t chunk = gd_chunk_len(n_tail);
+    const size_t nchunks = (n_tail + chunk - 1) / chunk;
     size_t gh = (n_head + GD_BLOCK - 1) / GD_BLOCK;
     const size_t gcap = (size_t)(h->num_cu > 0 ? h->num_cu : 256) * 4;
     if (gh > gcap) gh = gcap;
@@ -303,10 +319,10 @@ extern "C" int hpb_gendot_f64(hpb_handle_t h, size_t n, double log2cond, uint64_
     double* o3 = rl + nchunks;
     hipLaunchKernelGGL(k_head, dim3((unsigned)gh), dim3(GD_BLOCK), 0, h->stream, (uint64_t)n_head, hb, seed, x, y, hpart);
     hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(GD_BLOCK), 0, h->stream, (int)gh, hpart, acc);
-    hipLaunchKernelGGL(k_tail_chunks, dim3((unsigned)((nchunks + GD_BLOCK - 1) / GD_BLOCK)), dim3(GD_BLOCK), 0,
-                       h->stream, (uint64_t)n_head, (uint64_t)n_tail, hb, seed, acc, x, y, cpart, rl);
-    hipLaunchKernelGGL(k_tail_fix, dim3(1), dim3(64), 0, h->stream, (uint64_t)n_head, (uint64_t)n_tail, cpart, rl, x, y,
-                       acc, o3);
+    hipLaunchKernelGGL(k_tail_chunks, dim3((unsigned)nchunks), dim3(GD_BLOCK), 0, h->stream, (uint64_t)n_head,
+                       (uint64_t)n_tail, chunk, hb, seed, acc, x, y, cpart, rl);
+    hipLaunchKernelGGL(k_tail_fix, dim3(1), dim3(64), 0, h->stream, (uint64_t)n_head, (uint64_t)n_tail, chunk, cpart, rl,
+                       x, y, acc, o3);
     hipError_t le = hipGetLastError();
     uint64_t hacc[8];
     double ho[3];

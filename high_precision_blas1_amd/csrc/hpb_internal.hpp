@@ -28,6 +28,7 @@ struct hpb_handle_s {
     void* dres = nullptr;           // device results, HPB_DRES_BYTES
     void* hres = nullptr;           // pinned host mirror of dres
     int* dflag = nullptr;           // validity flag
+    void* nrmfix = nullptr;         // hpb::nrm_fix record of the scaled norms (zero at rest)
     hipEvent_t ev[2] = {nullptr, nullptr};
     hpb_tuning_t tune_red[2];       // two-input reductions (dot), [0] f64, [1] f32
     hpb_tuning_t tune_red1[2];      // one-input reductions (sum, asum, nrm2)
@@ -41,6 +42,16 @@ struct hpb_handle_s {
 #define HPB_MAX_MULTIDOT 64
 
 namespace hpb {
+
+// State passed from a norm launch to its rescaling launch (hpb_reduce.hip): need
+// is set when max |x_i| fell outside the range in which the unscaled sum of
+// squares can neither overflow nor lose its error terms to underflow; the
+// rescaling launch clears it.
+struct nrm_fix {
+    int need;
+    int pad;
+    double amax;
+};
 
 void set_error(const char* file, int line, const std::string& msg);
 int hip_status(hipError_t e, const char* file, int line, const char* what);

@@ -13,7 +13,14 @@
 //     block that takes the last ticket merges them in block-index order, so the
 //     result does not depend on which block finishes last (reproducible); no
 //     floating-point atomics, no in-place multi-pass (the reference's passes read
-//     and write the same array from different blocks, impl.cuh:1247-1253).
+//     and write the same array from different blocks, impl.cuh:1247-1253);
+//   * the norms are safe against overflow and underflow, as the reference's
+//     standard-precision norm (cuBLAS nrm2, gpu_vector_operations.h:315-327) is:
+//     the launch also takes M = max |x_i|; when M lies outside [lo, hi] (2^-400,
+//     2^400 in FP64; 2^-30, 2^30 in FP32) a second launch of the same shape,
+//     queued behind it, repeats the sum over x_i 2^k with the power of two 2^k
+//     that puts M in [1/2, 1), and scales the result back.  Inside the range the
+//     unscaled sum is kept, so those results are unchanged by the scaling.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -31,6 +38,13 @@ template <int OP> struct red_traits {
     static constexpr bool nrm = (OP == R_NRM2C || OP == R_NRM2);
 };
 
+// Range of M = max |x_i| in which the norms sum unscaled squares: n M^2 stays
+// finite for n < 2^48, and the TwoProd error terms of the largest squares stay
+// normal numbers.
+template <typename T> struct nrm_range;
+template <> struct nrm_range<double> { static constexpr double lo = 0x1p-400, hi = 0x1p400; };
+template <> struct nrm_range<float> { static constexpr double lo = 0x1p-30, hi = 0x1p30; };
+
 template <typename T> __device__ __forceinline__ T absv(T v);
 template <> __device__ __forceinline__ double absv<double>(double v) { return __builtin_fabs(v); }
 template <> __device__ __forceinline__ float absv<float>(float v) { return __builtin_fabsf(v); }
@@ -38,6 +52,32 @@ template <> __device__ __forceinline__ float absv<float>(float v) { return __bui
 template <typename T> __device__ __forceinline__ T sqrtv(T v);
 template <> __device__ __forceinline__ double sqrtv<double>(double v) { return __builtin_sqrt(v); }
 template <> __device__ __forceinline__ float sqrtv<float>(float v) { return __builtin_sqrtf(v); }
+
+template <typename T> __device__ __forceinline__ T maxv(T a, T b);
+template <> __device__ __forceinline__ double maxv<double>(double a, double b) { return __builtin_fmax(a, b); }
+template <> __device__ __forceinline__ float maxv<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+
+template <typename T> __device__ __forceinline__ T ldexpv(T v, int k);
+template <> __device__ __forceinline__ double ldexpv<double>(double v, int k) { return __builtin_ldexp(v, k); }
+template <> __device__ __forceinline__ float ldexpv<float>(float v, int k) { return __builtin_ldexpf(v, k); }
+
+// Block maximum (NaN ignored, as fmax does); result valid in thread 0.
+template <typename T, int BLOCK>
+__device__ __forceinline__ T block_max(T m)
+{
+    constexpr int NW = BLOCK / 64;
+    __shared__ T lds_m[NW];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = maxv<T>(m, dev::shfl_xor<T>(m, d));
+    if (NW > 1) {
+        if ((threadIdx.x & 63) == 0) lds_m[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int w = 1; w < NW; ++w) m = maxv<T>(m, lds_m[w]);
+        __syncthreads();
+    }
+    return m;
+}
 
 template <typename T, int OP>
 __device__ __forceinline__ void red_acc(T& s, T& c, T xv, T yv)
@@ -86,6 +126,40 @@ __device__ __forceinline__ void write_out(T* out, T s, T c)
     out[3] = red_traits<OP>::nrm ? sqrtv<T>(r) : r;
 }
 
+// Last stage of a norm launch (thread 0): the unscaled result, and the request for
+// the rescaling launch when M is outside the safe range (M = 0 and an empty x
+// need none; a NaN entry leaves M alone and makes the sum NaN).
+template <typename T, int OP>
+__device__ __forceinline__ void nrm_finish(T* out, T s, T c, T m, nrm_fix* fix)
+{
+    write_out<T, OP>(out, s, c);
+    const double md = (double)m;
+    if (md != 0.0 && !(md >= nrm_range<T>::lo && md <= nrm_range<T>::hi)) {
+        fix->amax = md;
+        fix->need = 1;
+    }
+}
+
+// Last stage of the rescaling launch (thread 0): (s, c) is the sum over x_i 2^k;
+// for M = inf, k = 0 and s alone is kept (inf, or NaN when x holds a NaN).
+template <typename T>
+__device__ __forceinline__ void nrm_rescaled_out(T* out, T s, T c, int k, bool inf_mode, nrm_fix* fix)
+{
+    if (inf_mode) {
+        out[0] = s;
+        out[1] = T(0);
+        out[2] = s;
+        out[3] = sqrtv<T>(s);
+    } else {
+        T r = s + c;
+        out[0] = ldexpv<T>(s, -2 * k);
+        out[1] = ldexpv<T>(c, -2 * k);
+        out[2] = ldexpv<T>(r, -2 * k);
+        out[3] = ldexpv<T>(sqrtv<T>(r), -k);
+    }
+    fix->need = 0;
+}
+
 // Fixed-order final stage, run by the block holding the last ticket: merges the
 // G partials in index order (thread t takes t, t+BLOCK, ...; then block_reduce).
 template <typename T, bool COMP, int BLOCK>
@@ -121,15 +195,34 @@ __device__ __forceinline__ bool take_ticket(unsigned* ticket, unsigned G)
 }
 
 // NT: stream the full tiles with nontemporal loads (no reuse is expected).
-template <typename T, int OP, int BLOCK, int UNROLL, bool VEC, bool NT = false>
+// RESCALE (norms only): the rescaling launch.  It does nothing unless the norm
+// launch before it asked for it (fix->need), then sums over x_i 2^k.
+template <typename T, int OP, int BLOCK, int UNROLL, bool VEC, bool NT = false, bool RESCALE = false>
 __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict__ x, const T* __restrict__ y,
                                                   T* __restrict__ part, unsigned* __restrict__ ticket,
-                                                  T* __restrict__ out, const int* __restrict__ skip)
+                                                  T* __restrict__ out, const int* __restrict__ skip,
+                                                  nrm_fix* __restrict__ fix)
 {
     using tr = red_traits<OP>;
+    static_assert(!RESCALE || tr::nrm, "only the norms are rescaled");
+    constexpr bool TRACK = tr::nrm && !RESCALE;   // take M = max |x_i| on the way
     // solver graphs pass a device flag: once it is set the launch does nothing
     // (every block returns, so the ticket is untouched)
     if (skip != nullptr && *skip) return;
+    int k = 0;
+    bool inf_mode = false;
+    if constexpr (RESCALE) {
+        // read by every block before any block can take the last ticket, so the
+        // clearing store of the last block cannot overtake it
+        if (fix->need == 0) return;
+        const double am = fix->amax;
+        if (am > 1.7976931348623157e308) {
+            inf_mode = true;
+        } else {
+            (void)__builtin_frexp(am, &k);   // am = f 2^k, f in [1/2, 1)
+            k = -k;
+        }
+    }
     using V = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
     constexpr int W = VEC ? dev::vec16<T>::W : 1;
     constexpr size_t TILE = (size_t)BLOCK * UNROLL;
@@ -142,7 +235,12 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict_
     const V* __restrict__ xv = reinterpret_cast<const V*>(x);
     const V* __restrict__ yv = reinterpret_cast<const V*>(y);
 
-    T s = T(0), c = T(0);
+    T s = T(0), c = T(0), m = T(0);
+    auto acc = [&](T xe, T ye) {
+        if constexpr (TRACK) m = maxv<T>(m, absv<T>(xe));
+        if constexpr (RESCALE) xe = ldexpv<T>(xe, k);
+        red_acc<T, OP>(s, c, xe, ye);
+    };
     for (size_t tile = b; tile < nfull; tile += G) {
         const size_t base = tile * TILE + tid;
         V xa[UNROLL], ya[UNROLL];
@@ -160,8 +258,8 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict_
         for (int u = 0; u < UNROLL; ++u) {
 #pragma unroll
             for (int w = 0; w < W; ++w) {
-                if constexpr (tr::binary) red_acc<T, OP>(s, c, lane_of<T>(xa[u], w), lane_of<T>(ya[u], w));
-                else red_acc<T, OP>(s, c, lane_of<T>(xa[u], w), T(0));
+                if constexpr (tr::binary) acc(lane_of<T>(xa[u], w), lane_of<T>(ya[u], w));
+                else acc(lane_of<T>(xa[u], w), T(0));
             }
         }
     }
@@ -177,29 +275,161 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict_
                 V bb = a;
                 if constexpr (tr::binary) bb = yv[vi];
 #pragma unroll
-                for (int w = 0; w < W; ++w) red_acc<T, OP>(s, c, lane_of<T>(a, w), lane_of<T>(bb, w));
+                for (int w = 0; w < W; ++w) acc(lane_of<T>(a, w), lane_of<T>(bb, w));
             }
         }
         if constexpr (VEC) {
             const size_t r0 = nv * W;
-            if (tid < n - r0) red_acc<T, OP>(s, c, x[r0 + tid], tr::binary ? y[r0 + tid] : T(0));
+            if (tid < n - r0) acc(x[r0 + tid], tr::binary ? y[r0 + tid] : T(0));
         }
     }
 
     dev::block_reduce<T, tr::comp, BLOCK>(s, c);
+    if constexpr (TRACK) m = block_max<T, BLOCK>(m);
     if (G == 1) {
-        if (tid == 0) write_out<T, OP>(out, s, c);
+        if (tid == 0) {
+            if constexpr (TRACK) nrm_finish<T, OP>(out, s, c, m, fix);
+            else if constexpr (RESCALE) nrm_rescaled_out<T>(out, s, c, k, inf_mode, fix);
+            else write_out<T, OP>(out, s, c);
+        }
         return;
     }
     if (tid == 0) {
         st_agent<T>(&part[2 * b], s);
         st_agent<T>(&part[2 * b + 1], c);
+        if constexpr (TRACK) st_agent<T>(&part[2 * G + b], m);
     }
     if (!take_ticket(ticket, G)) return;
     T fs, fc;
     final_merge<T, tr::comp, BLOCK>(part, G, fs, fc);
+    T fm = T(0);
+    if constexpr (TRACK) {
+        for (unsigned i = tid; i < G; i += BLOCK) fm = maxv<T>(fm, ld_agent<T>(&part[2 * G + i]));
+        fm = block_max<T, BLOCK>(fm);
+    }
     if (tid == 0) {
-        write_out<T, OP>(out, fs, fc);
+        if constexpr (TRACK) nrm_finish<T, OP>(out, fs, fc, fm, fix);
+        else if constexpr (RESCALE) nrm_rescaled_out<T>(out, fs, fc, k, inf_mode, fix);
+        else write_out<T, OP>(out, fs, fc);
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// min and max (gpu_reduction::min_max, source/common/gpu_reduction.h:61-68,
+// gpu_reduction_impl.cuh:82-264): NaN entries are skipped, as the reference's
+// comparisons skip them; -0 orders below +0, so the result does not depend on the
+// order of the merges.  Same launch shape, partials and last-block ticket as the
+// one-input reductions; out2 = { min, max }, { +inf, -inf } for an empty x.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T min_ord(T a, T b)
+{
+    if (b != b) return a;
+    if (a != a) return b;
+    if (a < b) return a;
+    if (b < a) return b;
+    return __builtin_signbit(a) ? a : b;
+}
+template <typename T>
+__device__ __forceinline__ T max_ord(T a, T b)
+{
+    if (b != b) return a;
+    if (a != a) return b;
+    if (a > b) return a;
+    if (b > a) return b;
+    return __builtin_signbit(a) ? b : a;
+}
+
+template <typename T, int BLOCK>
+__device__ __forceinline__ void block_minmax(T& lo, T& hi)
+{
+    constexpr int NW = BLOCK / 64;
+    __shared__ T lds_lo[NW], lds_hi[NW];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        lo = min_ord<T>(lo, dev::shfl_xor<T>(lo, d));
+        hi = max_ord<T>(hi, dev::shfl_xor<T>(hi, d));
+    }
+    if (NW > 1) {
+        if ((threadIdx.x & 63) == 0) {
+            lds_lo[threadIdx.x >> 6] = lo;
+            lds_hi[threadIdx.x >> 6] = hi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int w = 1; w < NW; ++w) {
+                lo = min_ord<T>(lo, lds_lo[w]);
+                hi = max_ord<T>(hi, lds_hi[w]);
+            }
+        __syncthreads();
+    }
+}
+
+template <typename T, int BLOCK, int UNROLL, bool VEC>
+__global__ __launch_bounds__(BLOCK) void k_minmax(size_t n, const T* __restrict__ x, T* __restrict__ part,
+                                                  unsigned* __restrict__ ticket, T* __restrict__ out)
+{
+    using V = std::conditional_t<VEC, typename dev::vec16<T>::type, T>;
+    constexpr int W = VEC ? dev::vec16<T>::W : 1;
+    constexpr size_t TILE = (size_t)BLOCK * UNROLL;
+    const unsigned G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
+    const size_t nv = n / W;
+    const size_t nfull = nv / TILE;
+    const V* __restrict__ xv = reinterpret_cast<const V*>(x);
+    T lo = __builtin_huge_val(), hi = -__builtin_huge_val();
+    auto acc = [&](T v) {
+        lo = min_ord<T>(lo, v);
+        hi = max_ord<T>(hi, v);
+    };
+    for (size_t tile = b; tile < nfull; tile += G) {
+        const size_t base = tile * TILE + tid;
+        V xa[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) xa[u] = xv[base + (size_t)u * BLOCK];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+            for (int w = 0; w < W; ++w) acc(lane_of<T>(xa[u], w));
+    }
+    if (b == (unsigned)(nfull % G)) {
+        const size_t base = nfull * TILE + tid;
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const size_t vi = base + (size_t)u * BLOCK;
+            if (vi < nv) {
+                V a = xv[vi];
+#pragma unroll
+                for (int w = 0; w < W; ++w) acc(lane_of<T>(a, w));
+            }
+        }
+        if constexpr (VEC) {
+            const size_t r0 = nv * W;
+            if (tid < n - r0) acc(x[r0 + tid]);
+        }
+    }
+    block_minmax<T, BLOCK>(lo, hi);
+    if (G == 1) {
+        if (tid == 0) {
+            out[0] = lo;
+            out[1] = hi;
+        }
+        return;
+    }
+    if (tid == 0) {
+        st_agent<T>(&part[2 * b], lo);
+        st_agent<T>(&part[2 * b + 1], hi);
+    }
+    if (!take_ticket(ticket, G)) return;
+    T flo = __builtin_huge_val(), fhi = -__builtin_huge_val();
+    for (unsigned i = tid; i < G; i += BLOCK) {
+        flo = min_ord<T>(flo, ld_agent<T>(&part[2 * i]));
+        fhi = max_ord<T>(fhi, ld_agent<T>(&part[2 * i + 1]));
+    }
+    block_minmax<T, BLOCK>(flo, fhi);
+    if (tid == 0) {
+        out[0] = flo;
+        out[1] = fhi;
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -233,34 +463,34 @@ static red_shape reduce_shape(const hpb_handle_s* h, size_t n, bool aligned, boo
     return r;
 }
 
-template <typename T, int OP, int BLOCK, int UNROLL>
+template <typename T, int OP, int BLOCK, int UNROLL, bool RS>
 static void launch_bu(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
-                      unsigned* ticket, T* out, const int* skip)
+                      unsigned* ticket, T* out, const int* skip, nrm_fix* fix)
 {
     if (sh.vecw > 1 && (sh.flags & HPB_TUNE_NONTEMPORAL) && UNROLL >= 2)
-        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true, true>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y,
-                           part, ticket, out, skip);
+        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true, true, RS>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x,
+                           y, part, ticket, out, skip, fix);
     else if (sh.vecw > 1)
-        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y, part,
-                           ticket, out, skip);
+        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true, false, RS>), dim3(sh.grid), dim3(BLOCK), 0, st, n,
+                           x, y, part, ticket, out, skip, fix);
     else
-        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, false>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x, y, part,
-                           ticket, out, skip);
+        hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, false, false, RS>), dim3(sh.grid), dim3(BLOCK), 0, st, n,
+                           x, y, part, ticket, out, skip, fix);
 }
 
-template <typename T, int OP>
+template <typename T, int OP, bool RS = false>
 static void launch_red(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
-                       unsigned* ticket, T* out, const int* skip)
+                       unsigned* ticket, T* out, const int* skip, nrm_fix* fix)
 {
     switch (sh.block * 16 + sh.unroll) {
-    case 256 * 16 + 1: launch_bu<T, OP, 256, 1>(sh, st, n, x, y, part, ticket, out, skip); break;
-    case 256 * 16 + 2: launch_bu<T, OP, 256, 2>(sh, st, n, x, y, part, ticket, out, skip); break;
-    case 256 * 16 + 8: launch_bu<T, OP, 256, 8>(sh, st, n, x, y, part, ticket, out, skip); break;
-    case 512 * 16 + 1: launch_bu<T, OP, 512, 1>(sh, st, n, x, y, part, ticket, out, skip); break;
-    case 512 * 16 + 2: launch_bu<T, OP, 512, 2>(sh, st, n, x, y, part, ticket, out, skip); break;
-    case 512 * 16 + 4: launch_bu<T, OP, 512, 4>(sh, st, n, x, y, part, ticket, out, skip); break;
-    case 512 * 16 + 8: launch_bu<T, OP, 512, 8>(sh, st, n, x, y, part, ticket, out, skip); break;
-    default: launch_bu<T, OP, 256, 4>(sh, st, n, x, y, part, ticket, out, skip); break;
+    case 256 * 16 + 1: launch_bu<T, OP, 256, 1, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    case 256 * 16 + 2: launch_bu<T, OP, 256, 2, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    case 256 * 16 + 8: launch_bu<T, OP, 256, 8, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    case 512 * 16 + 1: launch_bu<T, OP, 512, 1, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    case 512 * 16 + 2: launch_bu<T, OP, 512, 2, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    case 512 * 16 + 4: launch_bu<T, OP, 512, 4, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    case 512 * 16 + 8: launch_bu<T, OP, 512, 8, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    default: launch_bu<T, OP, 256, 4, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
     }
 }
 
@@ -280,14 +510,63 @@ static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev
     const bool al = aligned16(x) && (!red_traits<OP>::binary || aligned16(y));
     red_shape sh = reduce_shape<T>(h, n, al, red_traits<OP>::binary);
     T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
-    launch_red<T, OP>(sh, h->stream, n, x, y, reinterpret_cast<T*>(h->part), h->ticket, out, skip);
+    T* part = reinterpret_cast<T*>(h->part);
+    nrm_fix* fix = reinterpret_cast<nrm_fix*>(h->nrmfix);
+    launch_red<T, OP>(sh, h->stream, n, x, y, part, h->ticket, out, skip, fix);
     HPB_LAUNCH_CHECK();
+    if constexpr (red_traits<OP>::nrm) {
+        // asynchronous: the rescaling launch is always queued (it returns at once
+        // unless the norm launch asked for it); blocking: only when asked
+        if (dev_out) {
+            launch_red<T, OP, true>(sh, h->stream, n, x, y, part, h->ticket, out, skip, fix);
+            HPB_LAUNCH_CHECK();
+            return HPB_OK;
+        }
+        HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+        HPB_HIP(hipMemcpyAsync(reinterpret_cast<char*>(h->hres) + 64, h->nrmfix, sizeof(nrm_fix),
+                               hipMemcpyDeviceToHost, h->stream));
+        HPB_HIP(hipStreamSynchronize(h->stream));
+        const nrm_fix* hf = reinterpret_cast<const nrm_fix*>(reinterpret_cast<const char*>(h->hres) + 64);
+        if (hf->need) {
+            launch_red<T, OP, true>(sh, h->stream, n, x, y, part, h->ticket, out, skip, fix);
+            HPB_LAUNCH_CHECK();
+            HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+            HPB_HIP(hipStreamSynchronize(h->stream));
+        }
+        const T* r = reinterpret_cast<const T*>(h->hres);
+        for (int i = 0; i < 4; ++i) host_out[i] = r[i];
+        return HPB_OK;
+    }
     if (!dev_out) {
         HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T), hipMemcpyDeviceToHost, h->stream));
         HPB_HIP(hipStreamSynchronize(h->stream));
         const T* r = reinterpret_cast<const T*>(h->hres);
         for (int i = 0; i < 4; ++i) host_out[i] = r[i];
     }
+    return HPB_OK;
+}
+
+template <typename T>
+static int minmax_entry(hpb_handle_t h, size_t n, const T* x, T* host_out2)
+{
+    HPB_CLEAR_STALE();
+    HPB_REQUIRE(h && host_out2 && (n == 0 || x), HPB_ERR_INVALID, "hpb_minmax: null argument");
+    const bool al = aligned16(x);
+    // 512 threads x 4 16-byte vectors per tile, at most one block per CU
+    const size_t nv = n / (size_t)(al ? dev::vec16<T>::W : 1);
+    size_t grid = (nv + 2047) / 2048;
+    const size_t gmax = h->num_cu > 0 ? (size_t)h->num_cu : 256;
+    grid = grid < 1 ? 1 : (grid > gmax ? gmax : grid);
+    T* out = reinterpret_cast<T*>(h->dres);
+    T* part = reinterpret_cast<T*>(h->part);
+    const dim3 g((unsigned)grid), blk(512);
+    if (al) hipLaunchKernelGGL((k_minmax<T, 512, 4, true>), g, blk, 0, h->stream, n, x, part, h->ticket, out);
+    else hipLaunchKernelGGL((k_minmax<T, 512, 4, false>), g, blk, 0, h->stream, n, x, part, h->ticket, out);
+    HPB_LAUNCH_CHECK();
+    HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 2 * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    HPB_HIP(hipStreamSynchronize(h->stream));
+    host_out2[0] = reinterpret_cast<const T*>(h->hres)[0];
+    host_out2[1] = reinterpret_cast<const T*>(h->hres)[1];
     return HPB_OK;
 }
 
@@ -380,5 +659,14 @@ HPB_RED_UN(nrm2c, R_NRM2C)
 HPB_RED_UN(nrm2, R_NRM2)
 HPB_RED_UN_ASYNC(nrm2c, R_NRM2C)
 HPB_RED_UN_ASYNC(nrm2, R_NRM2)
+
+int hpb_minmax_f64(hpb_handle_t h, size_t n, const double* x, double* out2)
+{
+    return minmax_entry<double>(h, n, x, out2);
+}
+int hpb_minmax_f32(hpb_handle_t h, size_t n, const float* x, float* out2)
+{
+    return minmax_entry<float>(h, n, x, out2);
+}
 
 }  // extern "C"

@@ -6,6 +6,10 @@
 //   lower: L x = alpha b with the strictly lower triangle and a unit diagonal
 //          (CUSPARSE_DIAG_TYPE_UNIT: the stored diagonal is ignored);
 //   upper: U x = alpha b with the upper triangle including the stored diagonal.
+// The transposed solves of the reference (L^T x = alpha b, U^T x = alpha b) are the
+// upper solve with a unit diagonal and the lower solve with the stored diagonal on
+// the transpose of the matrix (hpb_csr_transpose_*); the analysis therefore also
+// takes those two kinds (uplo 2: lower, stored diagonal; 3: upper, unit diagonal).
 // Both read only their triangle, so one CSR holding an incomplete LU factorisation
 // (L strictly below, U on and above the diagonal) serves both solves, as in the
 // reference's csr::preconditioner (common/csr/preconditioner.h:61-75).
@@ -33,7 +37,9 @@
 #include "hpb_internal.hpp"
 
 struct hpb_trsv_s {
-    int uplo = 0;        // 0 lower (unit diagonal), 1 upper
+    int uplo = 0;        // 0 lower (unit diagonal), 1 upper, 2 lower (stored diagonal), 3 upper (unit diagonal)
+    bool upper = false;
+    bool unit = true;
     int rows = 0;
     int64_t nnz = 0;
     const int* rp = nullptr;    // the caller's device arrays (not owned)
@@ -50,7 +56,7 @@ namespace {
 
 constexpr int TB = 256;
 
-template <typename T, int LPR, bool COMP, bool UPPER>
+template <typename T, int LPR, bool COMP, bool UPPER, bool UNIT>
 __global__ __launch_bounds__(TB) void k_trsv(int beg, int end, const int* __restrict__ perm,
                                              const int* __restrict__ rp, const int* __restrict__ ci,
                                              const T* __restrict__ val, const int* __restrict__ diag, T alpha,
@@ -82,7 +88,7 @@ __global__ __launch_bounds__(TB) void k_trsv(int beg, int end, const int* __rest
     }
     if (lane == 0) {
         T v = COMP ? s + c : s;
-        if (UPPER) v = v / val[diag[i]];
+        if (!UNIT) v = v / val[diag[i]];
         x[i] = v;
     }
 }
@@ -115,28 +121,28 @@ __global__ __launch_bounds__(64) void k_ilu0_rows(int beg, const int* __restrict
     }
 }
 
-template <typename T, bool COMP, bool UPPER, int LPR>
+template <typename T, bool COMP, bool UPPER, bool UNIT, int LPR>
 void trsv_level(hipStream_t st, const hpb_trsv_s* t, int beg, int end, const T* val, T alpha, const T* b, T* x)
 {
     const int rows = end - beg;
     const int per_block = TB / LPR;
     const unsigned grid = (unsigned)((rows + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_trsv<T, LPR, COMP, UPPER>), dim3(grid), dim3(TB), 0, st, beg, end, t->perm, t->rp, t->ci,
-                       val, t->diag, alpha, b, x);
+    hipLaunchKernelGGL((k_trsv<T, LPR, COMP, UPPER, UNIT>), dim3(grid), dim3(TB), 0, st, beg, end, t->perm, t->rp,
+                       t->ci, val, t->diag, alpha, b, x);
 }
 
-template <typename T, bool COMP, bool UPPER>
+template <typename T, bool COMP, bool UPPER, bool UNIT>
 void trsv_all(hipStream_t st, const hpb_trsv_s* t, const T* val, T alpha, const T* b, T* x)
 {
     const int nl = (int)t->level_ptr.size() - 1;
     for (int l = 0; l < nl; ++l) {
         const int beg = t->level_ptr[l], end = t->level_ptr[l + 1];
         switch (t->lpr) {
-        case 1: trsv_level<T, COMP, UPPER, 1>(st, t, beg, end, val, alpha, b, x); break;
-        case 2: trsv_level<T, COMP, UPPER, 2>(st, t, beg, end, val, alpha, b, x); break;
-        case 4: trsv_level<T, COMP, UPPER, 4>(st, t, beg, end, val, alpha, b, x); break;
-        case 8: trsv_level<T, COMP, UPPER, 8>(st, t, beg, end, val, alpha, b, x); break;
-        default: trsv_level<T, COMP, UPPER, 16>(st, t, beg, end, val, alpha, b, x); break;
+        case 1: trsv_level<T, COMP, UPPER, UNIT, 1>(st, t, beg, end, val, alpha, b, x); break;
+        case 2: trsv_level<T, COMP, UPPER, UNIT, 2>(st, t, beg, end, val, alpha, b, x); break;
+        case 4: trsv_level<T, COMP, UPPER, UNIT, 4>(st, t, beg, end, val, alpha, b, x); break;
+        case 8: trsv_level<T, COMP, UPPER, UNIT, 8>(st, t, beg, end, val, alpha, b, x); break;
+        default: trsv_level<T, COMP, UPPER, UNIT, 16>(st, t, beg, end, val, alpha, b, x); break;
         }
     }
 }
@@ -146,12 +152,24 @@ int trsv_entry(hpb_handle_t h, hpb_trsv_t t, const T* val, T alpha, const T* b, 
 {
     HPB_CLEAR_STALE();
     HPB_REQUIRE(h && t && val && b && x, HPB_ERR_INVALID, "hpb_csr_trsv: null argument");
-    if (t->uplo == 0) {
-        if (comp) trsv_all<T, true, false>(h->stream, t, val, alpha, b, x);
-        else trsv_all<T, false, false>(h->stream, t, val, alpha, b, x);
-    } else {
-        if (comp) trsv_all<T, true, true>(h->stream, t, val, alpha, b, x);
-        else trsv_all<T, false, true>(h->stream, t, val, alpha, b, x);
+    hipStream_t st = h->stream;
+    switch (t->uplo) {
+    case 0:
+        if (comp) trsv_all<T, true, false, true>(st, t, val, alpha, b, x);
+        else trsv_all<T, false, false, true>(st, t, val, alpha, b, x);
+        break;
+    case 1:
+        if (comp) trsv_all<T, true, true, false>(st, t, val, alpha, b, x);
+        else trsv_all<T, false, true, false>(st, t, val, alpha, b, x);
+        break;
+    case 2:
+        if (comp) trsv_all<T, true, false, false>(st, t, val, alpha, b, x);
+        else trsv_all<T, false, false, false>(st, t, val, alpha, b, x);
+        break;
+    default:
+        if (comp) trsv_all<T, true, true, true>(st, t, val, alpha, b, x);
+        else trsv_all<T, false, true, true>(st, t, val, alpha, b, x);
+        break;
     }
     HPB_LAUNCH_CHECK();
     return HPB_OK;
@@ -186,7 +204,8 @@ int hpb_csr_trsv_analysis(hpb_handle_t h, int uplo, int rows, int64_t nnz, const
 {
     HPB_CLEAR_STALE();
     HPB_REQUIRE(h && row_ptr && col_ind && out, HPB_ERR_INVALID, "hpb_csr_trsv_analysis: null argument");
-    HPB_REQUIRE(uplo == 0 || uplo == 1, HPB_ERR_INVALID, "hpb_csr_trsv_analysis: uplo must be 0 or 1");
+    HPB_REQUIRE(uplo >= 0 && uplo <= 3, HPB_ERR_INVALID, "hpb_csr_trsv_analysis: uplo must be 0, 1, 2 or 3");
+    const bool lower = (uplo == 0 || uplo == 2), unit = (uplo == 0 || uplo == 3);
     HPB_REQUIRE(rows > 0 && nnz > 0 && nnz < (1ll << 31), HPB_ERR_INVALID, "hpb_csr_trsv_analysis: bad size");
     *out = nullptr;
     std::vector<int> rp((size_t)rows + 1), ci((size_t)nnz);
@@ -197,7 +216,7 @@ int hpb_csr_trsv_analysis(hpb_handle_t h, int uplo, int rows, int64_t nnz, const
     std::vector<int> level((size_t)rows, 0), diag((size_t)rows, -1);
     int nlev = 0;
     for (int s = 0; s < rows; ++s) {
-        const int i = uplo == 0 ? s : rows - 1 - s;   // dependency order
+        const int i = lower ? s : rows - 1 - s;   // dependency order
         int lv = 0;
         for (int k = rp[i]; k < rp[i + 1]; ++k) {
             const int j = ci[k];
@@ -205,15 +224,17 @@ int hpb_csr_trsv_analysis(hpb_handle_t h, int uplo, int rows, int64_t nnz, const
             HPB_REQUIRE(k == rp[i] || ci[k - 1] < j, HPB_ERR_INVALID,
                         "hpb_csr_trsv_analysis: columns must be sorted and unique within a row");
             if (j == i) diag[i] = k;
-            else if (uplo == 0 ? j < i : j > i) lv = std::max(lv, level[j] + 1);
+            else if (lower ? j < i : j > i) lv = std::max(lv, level[j] + 1);
         }
         level[i] = lv;
         nlev = std::max(nlev, lv + 1);
-        HPB_REQUIRE(uplo == 0 || diag[i] >= 0, HPB_ERR_INVALID,
-                    "hpb_csr_trsv_analysis: upper triangle without a stored diagonal entry (structurally singular)");
+        HPB_REQUIRE(unit || diag[i] >= 0, HPB_ERR_INVALID,
+                    "hpb_csr_trsv_analysis: a row without a stored diagonal entry (structurally singular)");
     }
     hpb_trsv_s* t = new hpb_trsv_s();
     t->uplo = uplo;
+    t->upper = !lower;
+    t->unit = unit;
     t->rows = rows;
     t->nnz = nnz;
     t->rp = row_ptr;

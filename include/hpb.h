@@ -102,7 +102,16 @@ int hpb_sum2_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
 int hpb_asum2_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
 int hpb_asum2_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
 /* Compensated 2-norm: gpu_reduction_ogita::norm = sqrt(Dot2(x,x)) (gpu_reduction_ogita.h:91-96),
- * reading x once; out4[3] = sqrt(fl(s + c)). */
+ * reading x once; out4[3] = sqrt(fl(s + c)).  Safe against overflow and underflow
+ * like the reference's standard-precision norm (cuBLAS nrm2, gpu_vector_operations.h:
+ * 315-327): while M = max |x_i| lies in [2^-400, 2^400] (FP64) / [2^-30, 2^30] (FP32)
+ * the squares are summed unscaled; otherwise a second pass sums (x_i 2^k)^2 with
+ * the power of two that puts M in [1/2, 1) and out4 = { s 2^-2k, c 2^-2k,
+ * fl(s + c) 2^-2k, sqrt(fl(s + c)) 2^-k } (the first three are the sum of squares,
+ * which may itself overflow or underflow; out4[3] is the norm).  An inf entry gives
+ * inf, a NaN entry NaN.  The blocking call adds the second pass only when needed;
+ * the _async call always queues it (it returns at once when not needed).  The
+ * plain hpb_nrm2_* follow the same rule. */
 int hpb_nrm2c_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
 int hpb_nrm2c_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
 int hpb_nrm2c_f64_async(hpb_handle_t h, size_t n, const double* x, double* dev_out4);
@@ -136,6 +145,13 @@ int hpb_nrm2_f64(hpb_handle_t h, size_t n, const double* x, double* out4);
 int hpb_nrm2_f32(hpb_handle_t h, size_t n, const float* x, float* out4);
 int hpb_nrm2_f64_async(hpb_handle_t h, size_t n, const double* x, double* dev_out4);
 int hpb_nrm2_f32_async(hpb_handle_t h, size_t n, const float* x, float* dev_out4);
+/* min and max: gpu_reduction::min_max (gpu_reduction.h:61-68, gpu_reduction_impl.cuh:
+ * 82-264).  NaN entries are skipped (as the reference's comparisons skip them);
+ * -0 counts below +0.  out2 = { min, max }; { +inf, -inf } for n = 0 (the
+ * reference starts from the integer 0x7f800000 converted to T, 2139095040, rather
+ * than infinity).  Blocking. */
+int hpb_minmax_f64(hpb_handle_t h, size_t n, const double* x, double* out2);
+int hpb_minmax_f32(hpb_handle_t h, size_t n, const float* x, float* out2);
 
 /* ---- elementwise (gpu_vector_operations_impl.cuh:226-528) ---------------- */
 /* Each *_f64 has an *_f32 twin with float scalars and pointers. */
@@ -195,9 +211,11 @@ int hpb_div_pointwise_inplace_f32(hpb_handle_t h, size_t n, float* x, float b, c
 
 /* ---- compensated elementwise (new: no reference counterpart; the nearest is
  *      add_mul, gpu_vector_operations.h:459-470, impl.cuh:352-383) ---------- */
-/* z := a*x + b*y evaluated error-free (TwoProd, TwoProd, TwoSum) and rounded
- * once at the end: faithful, and correctly rounded unless the two products
- * nearly cancel.  z == y or z == x allowed. */
+/* z := a*x + b*y from the error-free split a x + b y = s + t + e1 + e2 (TwoProd,
+ * TwoProd, TwoSum); z = s + ((e1 + e2) + t), where the correction term is rounded
+ * twice before the final addition: z is faithful (one of the two floating-point
+ * numbers around the exact value), not always correctly rounded.  z == y or
+ * z == x allowed. */
 int hpb_axpbyc_f64(hpb_handle_t h, size_t n, double a, const double* x, double b, const double* y, double* z);
 int hpb_axpbyc_f32(hpb_handle_t h, size_t n, float a, const float* x, float b, const float* y, float* z);
 
@@ -303,6 +321,20 @@ int hpb_lognormal_field_f64(hpb_handle_t h, int64_t ncell, double sigma, uint64_
  * diagonal = sum of the four faces ((s + w) + e) + n. */
 int hpb_csr_diffusion2d_f64(hpb_handle_t h, int m, const double* kappa, int* row_ptr, int* col_ind, double* val);
 
+/* ---- CSR transpose ---------------------------------------------------------- */
+/* B = A^T as CSR (cols x rows) into caller arrays t_row_ptr[cols + 1],
+ * t_col_ind[nnz], t_val[nnz], for the transposed products of csr::gpu_matrix::axpy
+ * (transpose = true, common/csr/gpu_matrix.h:322-333, cuSPARSE
+ * CUSPARSE_OPERATION_TRANSPOSE) and the transposed triangular solves: the ordinary
+ * row kernels then run on B.  Deterministic (rows of B sorted by column, then by
+ * value bits for duplicate entries); no floating-point atomics.  A setup call:
+ * allocates a temporary of about 4 (cols + 1) bytes and synchronises the stream.
+ * Requires nnz < 2^31 and column indices in [0, cols) (else HPB_ERR_INVALID). */
+int hpb_csr_transpose_f64(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                          const double* val, int* t_row_ptr, int* t_col_ind, double* t_val);
+int hpb_csr_transpose_f32(hpb_handle_t h, int rows, int cols, int64_t nnz, const int* row_ptr, const int* col_ind,
+                          const float* val, int* t_row_ptr, int* t_col_ind, float* t_val);
+
 /* ---- sparse triangular solves and ILU(0) ---------------------------------- */
 /* csr::gpu_matrix::triangular_solve_lower / _upper (common/csr/gpu_matrix.h:142-320,
  * cuSPARSE csrsv2 with an analysis and a cudaMalloc per call).  The analysis is done
@@ -312,7 +344,11 @@ int hpb_csr_diffusion2d_f64(hpb_handle_t h, int m, const double* kappa, int* row
  *   uplo 0: L x = alpha b with the strictly lower triangle and a unit diagonal
  *           (the stored diagonal is ignored, CUSPARSE_DIAG_TYPE_UNIT);
  *   uplo 1: U x = alpha b with the upper triangle and its stored diagonal (every
- *           row must store one).
+ *           row must store one);
+ *   uplo 2: lower triangle with its stored diagonal; uplo 3: strictly upper triangle
+ *           with a unit diagonal.  On the transpose B = A^T (hpb_csr_transpose_*)
+ *           these are the reference's transposed solves L^T x = alpha b (uplo 3 on B)
+ *           and U^T x = alpha b (uplo 2 on B), gpu_matrix.h:142-154,232-244.
  * One kernel per level; compensated = 1 evaluates each row's sum with Dot2. x may
  * alias b. */
 typedef struct hpb_trsv_s* hpb_trsv_t;

@@ -81,6 +81,10 @@ def lib():
          [ctypes.c_int, _sz, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp])
     _sig(L, "orc_hpb_order_reduce_f32", ctypes.c_int,
          [ctypes.c_int, _sz, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp])
+    _sig(L, "orc_hpb_order_nrm2", ctypes.c_int,
+         [ctypes.c_int, _sz, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp])
+    _sig(L, "orc_hpb_order_nrm2_f32", ctypes.c_int,
+         [ctypes.c_int, _sz, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp])
     _sig(L, "orc_zdot_order", ctypes.c_int,
          [ctypes.c_int, _sz, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp])
     _sig(L, "orc_zdot_order_f32", ctypes.c_int,
@@ -229,6 +233,19 @@ def hpb_order_reduce(op, x, y=None, cfg=(256, 4, 1, 2)):
     rc = f(REDUCE_OPS[op], x.size, ptr(x), ptr(y), int(cfg[0]), int(cfg[1]), int(cfg[2]), int(cfg[3]), ptr(out))
     if rc:
         raise ValueError("bad launch shape %r" % (cfg,))
+    return out
+
+
+def hpb_order_nrm2(op, x, cfg=(256, 4, 1, 2)):
+    """(s, c, s+c, norm) of the library's overflow- and underflow-safe norm ``op``
+    ("nrm2c" or "nrm2"): unscaled when max |x_i| lies in the safe range, else the
+    same order over x_i 2^k, scaled back."""
+    x = np.ascontiguousarray(x)
+    out = np.zeros(4, dtype=x.dtype)
+    f = lib().orc_hpb_order_nrm2 if x.dtype == np.float64 else lib().orc_hpb_order_nrm2_f32
+    rc = f(REDUCE_OPS[op], x.size, ptr(x), int(cfg[0]), int(cfg[1]), int(cfg[2]), int(cfg[3]), ptr(out))
+    if rc:
+        raise ValueError("bad op or launch shape %r %r" % (op, cfg))
     return out
 
 

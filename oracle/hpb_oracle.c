@@ -658,7 +658,8 @@ ORC_API void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2
  *   Every x and y is a multiple of 2^-53 (grid53 below), so every product is a
  *   multiple of 2^-106 and the running dot is held exactly in a 256-bit two's
  *   complement fixed-point accumulator with LSB 2^-106.
- *   Tail chunks of GD_CHUNK elements.  For a tail element that is not the last
+ *   Tail chunks of GD_CHUNK_MIN = 256 elements, the length doubled until there are
+ *   at most GD_MAX_CHUNKS = 4096 chunks.  For a tail element that is not the last
  *   of its chunk: y = grid53(fl(fl(r - r_prev) / x)), r_prev = target of the
  *   previous element, or fl(S_head) for the first tail element.  The last element
  *   of each chunk cancels the exact running sum S (head + all earlier elements):
@@ -666,7 +667,15 @@ ORC_API void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2
  *   grid53(v) = v if |v| >= 1/2 else rint(v 2^53) 2^-53.
  */
 
-#define GD_CHUNK 256
+#define GD_CHUNK_MIN 256
+#define GD_MAX_CHUNKS 4096
+
+static size_t gd_chunk_len(size_t n_tail)
+{
+    size_t c = GD_CHUNK_MIN;
+    while ((n_tail + c - 1) / c > GD_MAX_CHUNKS) c *= 2;
+    return c;
+}
 
 typedef struct { uint64_t w[4]; } acc256_t; /* two's complement, LSB 2^-106 */
 
@@ -798,9 +807,10 @@ ORC_API int orc_gendot(size_t n, double log2cond, uint64_t seed, size_t n_tail,
         acc_add_double(&A, fabs(p)); acc_add_double(&A, p < 0 ? -pe : pe);
     }
     double r_prev = acc_round(&S);
-    size_t nchunks = (n_tail + GD_CHUNK - 1) / GD_CHUNK;
+    size_t chunk = gd_chunk_len(n_tail);
+    size_t nchunks = (n_tail + chunk - 1) / chunk;
     for (size_t ch = 0; ch < nchunks; ++ch) {
-        size_t j0 = ch * GD_CHUNK, j1 = j0 + GD_CHUNK;
+        size_t j0 = ch * chunk, j1 = j0 + chunk;
         if (j1 > n_tail) j1 = n_tail;
         for (size_t j = j0; j < j1; ++j) {
             size_t i = n_head + j;
@@ -986,6 +996,50 @@ static inline int hp_comp(int op) { return op == 0 || op == 2 || op == 4 || op =
 
 HP_DEFINE(double, , two_sum, two_prod, fma, fabs)
 HP_DEFINE(float, _f32, two_sumf, two_prodf, fmaf, fabsf)
+
+/* The library's overflow- and underflow-safe norms (hpb_reduce.hip), op 6 nrm2c or
+ * 7 nrm2.  M = max |x_i| with NaN entries skipped (fmax).  M = 0, or M in
+ * [lo, hi] (2^-400, 2^400 for FP64; 2^-30, 2^30 for FP32): the unscaled result in
+ * the library's order, out4 = { s, c, s + c, sqrt(s + c) }.  Otherwise the same
+ * order over x_i 2^k, with 2^k the power of two that puts M in [1/2, 1)
+ * (ldexp: exact, or rounded once where an entry underflows), scaled back:
+ * { s 2^-2k, c 2^-2k, (s + c) 2^-2k, sqrt(s + c) 2^-k }.  M = inf: k = 0 and
+ * { s, 0, s, sqrt(s) } (inf, or NaN when x holds a NaN). */
+#define NRM_DEFINE(T, SUF, FMAX, FABS, LDEXP, SQRT, LO, HI)                                        \
+ORC_API int orc_hpb_order_nrm2##SUF(int op, size_t n, const T *x, int block, int unroll, int grid, \
+                                    int vecw, T *out4)                                            \
+{                                                                                                 \
+    if (op != 6 && op != 7) return 2;                                                             \
+    T m = 0;                                                                                      \
+    for (size_t i = 0; i < n; ++i) m = FMAX(m, FABS(x[i]));                                       \
+    T r3[3];                                                                                      \
+    if (m == 0 || ((double)m >= (LO) && (double)m <= (HI))) {                                     \
+        int rc = orc_hpb_order_reduce##SUF(op, n, x, x, block, unroll, grid, vecw, r3);           \
+        if (rc) return rc;                                                                        \
+        out4[0] = r3[0]; out4[1] = r3[1]; out4[2] = r3[2]; out4[3] = SQRT(r3[2]);                 \
+        return 0;                                                                                 \
+    }                                                                                             \
+    int inf_mode = isinf(m), k = 0;                                                               \
+    if (!inf_mode) { (void)frexp((double)m, &k); k = -k; }                                        \
+    T *xs = (T *)malloc((n ? n : 1) * sizeof(T));                                                 \
+    if (!xs) return 3;                                                                            \
+    for (size_t i = 0; i < n; ++i) xs[i] = LDEXP(x[i], k);                                        \
+    int rc = orc_hpb_order_reduce##SUF(op, n, xs, xs, block, unroll, grid, vecw, r3);             \
+    free(xs);                                                                                     \
+    if (rc) return rc;                                                                            \
+    if (inf_mode) {                                                                               \
+        out4[0] = r3[0]; out4[1] = 0; out4[2] = r3[0]; out4[3] = SQRT(r3[0]);                     \
+    } else {                                                                                      \
+        out4[0] = LDEXP(r3[0], -2 * k);                                                           \
+        out4[1] = LDEXP(r3[1], -2 * k);                                                           \
+        out4[2] = LDEXP(r3[0] + r3[1], -2 * k);                                                   \
+        out4[3] = LDEXP(SQRT(r3[0] + r3[1]), -k);                                                 \
+    }                                                                                             \
+    return 0;                                                                                     \
+}
+
+NRM_DEFINE(double, , fmax, fabs, ldexp, sqrt, 0x1p-400, 0x1p400)
+NRM_DEFINE(float, _f32, fmaxf, fabsf, ldexpf, sqrtf, 0x1p-30, 0x1p30)
 
 /* ------------------------------------------------------------------------ */
 /* Complex dot x^H y in the library's order (hpb_zdot.hip)                   */

@@ -35,6 +35,8 @@ def test_bindings_load():
     L = H.lib()
     assert L.hpb_version().decode().startswith("hpb")
     assert isinstance(L.hpb_last_error(), bytes)
+    from high_precision_blas1_amd._lib import missing_symbols
+    assert missing_symbols() == []
     protos = H.prototypes()
     for name in protos:
         assert getattr(L, name).argtypes is not None

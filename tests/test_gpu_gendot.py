@@ -26,7 +26,7 @@ def device_gendot(handle, n, cond, seed, n_tail):
 
 
 @pytest.mark.parametrize("n,n_tail", [(2, 1), (1000, 500), (4096, 1), (70001, 35000), ((1 << 20), 1 << 19),
-                                      (300000, 299000)])
+                                      (300000, 299000), ((1 << 21) + 3, (1 << 20) + (1 << 19) + 1)])
 @pytest.mark.parametrize("cond", [1e4, 1e16, 1e32])
 def test_bitwise_equal_to_host(handle, n, n_tail, cond):
     say("gendot", n, n_tail, cond)

@@ -203,3 +203,82 @@ def test_kahan_bound(rng):
     ax = float(np.abs(x).sum())
     k = O.lib().orc_sum_kahan(n, O.ptr(x))
     assert abs(F(k) - ex) <= F(2 * U * ax + n * U * U * ax) + F(U) * abs(ex)
+
+
+OPS_ALL = ["dot2", "dot", "sum2", "sum", "asum2", "asum", "nrm2c", "nrm2"]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("op", OPS_ALL)
+def test_order_reduce_every_op_exact_on_integers(rng, dtype, op):
+    """The library-order emulation for every op and both precisions: small integers
+    make every product and partial sum exact (below 2^24 in FP32), so the result
+    is the integer one in any order, and a dropped, repeated or misindexed element
+    changes it.  Covers the FP32 instantiation and the vector widths 1, 2 and 4."""
+    n = 50001
+    x = rng.integers(-8, 9, n).astype(dtype)
+    y = rng.integers(-8, 9, n).astype(dtype)
+    y[n // 3] = 3                      # the planted change below must matter
+    xi, yi = x.astype(np.int64), y.astype(np.int64)
+    want = {"dot2": int(xi @ yi), "dot": int(xi @ yi), "sum2": int(xi.sum()), "sum": int(xi.sum()),
+            "asum2": int(np.abs(xi).sum()), "asum": int(np.abs(xi).sum()), "nrm2c": int(xi @ xi),
+            "nrm2": int(xi @ xi)}[op]
+    assert abs(want) < 2 ** 24
+    binary = op in ("dot2", "dot")
+    for cfg in ((256, 4, 1, 1), (256, 4, 7, 2), (512, 2, 33, 4), (64, 1, 5, 1), (256, 8, 4096, 2),
+                (512, 1, 3, 4), (128, 2, 200, 2)):
+        out = O.hpb_order_reduce(op, x, y if binary else None, cfg=cfg)
+        assert out.dtype == dtype
+        assert out[2] == want, (op, cfg, out, want)
+        if O.REDUCE_OPS[op] in (0, 2, 4, 6):
+            assert out[1] == 0          # exact sums leave no compensation
+    # a planted mistake shows: one element changed by one
+    x2 = x.copy()
+    x2[n // 3] += 1
+    assert O.hpb_order_reduce(op, x2, y if binary else None, cfg=(256, 4, 7, 2))[2] != want
+
+
+def test_order_nrm2_scaling_rule():
+    """oracle.hpb_order_nrm2: unchanged inside the safe range, exact power-of-two
+    scaling outside it, LAPACK-style results at the extremes."""
+    import scipy.linalg.blas as B
+    r = np.random.default_rng(11)
+    base = r.uniform(-1, 1, 20011)
+    cfg = (512, 4, 64, 2)
+    # inside the range: identical to the unscaled library order
+    for op in ("nrm2c", "nrm2"):
+        a = O.hpb_order_nrm2(op, base, cfg)
+        b = O.hpb_order_reduce(op, base, cfg=cfg)
+        assert a[:3].tobytes() == b.tobytes() and a[3] == np.sqrt(b[2])
+    # outside it: the norm of the unscaled data times the power-of-two scale, bit
+    # for bit (same significands), and within 2 ulps of the scaled BLAS nrm2
+    n0 = O.hpb_order_nrm2("nrm2c", base, cfg)[3]
+    for e in (500, 700, 1000, -500, -700, -1000):
+        v = np.ldexp(base, e)
+        got = O.hpb_order_nrm2("nrm2c", v, cfg)
+        assert got[3] == np.ldexp(n0, e), e
+        assert abs(got[3] / B.dnrm2(v) - 1) <= 4.5e-16, e
+    # decimal scales change significands: compare with BLAS
+    for sc in (1e200, 1e-200, 1e300, 1e-300):
+        got = O.hpb_order_nrm2("nrm2c", base * sc, cfg)[3]
+        assert abs(got / B.dnrm2(base * sc) - 1) <= 4.5e-16, sc
+    # FP32
+    f = base.astype(np.float32)
+    f0 = O.hpb_order_nrm2("nrm2c", f, (512, 4, 64, 4))[3]
+    for e in (40, 60, -40, -60):
+        v = np.ldexp(f, e).astype(np.float32)
+        got = O.hpb_order_nrm2("nrm2c", v, (512, 4, 64, 4))[3]
+        assert got == np.float32(np.ldexp(np.float64(f0), e)), e
+        assert abs(float(got) / float(B.snrm2(v)) - 1) <= 2.5e-7, e
+    # closed form: 4^j copies of c have norm |c| 2^j
+    for e in (700, -700, 0):
+        v = np.full(1 << 16, 0.75 * 2.0 ** e)
+        assert O.hpb_order_nrm2("nrm2c", v, cfg)[3] == 0.75 * 2.0 ** e * 256
+    # special values: inf without NaN, NaN with one; zero and empty vectors
+    v = np.ones(1000)
+    v[3] = -np.inf
+    assert O.hpb_order_nrm2("nrm2c", v, cfg)[3] == np.inf
+    v[9] = np.nan
+    assert np.isnan(O.hpb_order_nrm2("nrm2c", v, cfg)[3])
+    assert O.hpb_order_nrm2("nrm2c", np.zeros(10), cfg)[3] == 0.0
+    assert O.hpb_order_nrm2("nrm2c", np.zeros(0), cfg)[3] == 0.0
