@@ -76,6 +76,7 @@ def gendot(H, h, n, cond, seed, x, y):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_reduce_dot2_n2p27_traffic.md")
+WARM_S = 1.0   # seconds of naive-dot launches before the timed windows
 
 
 def profiled_traffic(n, cfg, nontemporal):
@@ -535,7 +536,7 @@ def cfg_multidot(H, h, log):
     return {"n": nmax, "rows": rows, "length_sweep": sweep}
 
 
-def cfg_batch(H, h, log, nsys=512, recalc=0):
+def cfg_batch(H, h, log, nsys=512, recalc=0, use_graph=1):
     """512 independent 2-D 5-point diffusion systems, sides uniform in 64..1024 (numpy
     default_rng(3)), log-normal coefficients (sigma 1, seed 3), x* uniform in [-1, 1]
     (seed 7), BiCGStab(2) with Jacobi right preconditioning to 1e-10 true relative
@@ -559,7 +560,8 @@ def cfg_batch(H, h, log, nsys=512, recalc=0):
     rr = np.zeros(nsys)
     cv = np.zeros(nsys, dtype=np.int32)
     secs, cyc = ctypes.c_double(), ctypes.c_int()
-    H.call("hpb_batch_solve_bicgstab2", b, ctypes.c_double(1e-10), 40000, 1, 50, it.ctypes.data_as(ctypes.c_void_p),
+    H.call("hpb_batch_solve_bicgstab2", b, ctypes.c_double(1e-10), 40000, use_graph, 50,
+           it.ctypes.data_as(ctypes.c_void_p),
            rr.ctypes.data_as(ctypes.c_void_p), cv.ctypes.data_as(ctypes.c_void_p), ctypes.byref(secs),
            ctypes.byref(cyc))
     H.call("hpb_batch_destroy", b)
@@ -573,7 +575,8 @@ def cfg_batch(H, h, log, nsys=512, recalc=0):
             "build_s": round(t_build, 3),
             "solve_s": round(secs.value, 3), "cycles": cyc.value, "converged": int(cv.sum()),
             "iters_min_mean_max": [int(it.min()), round(float(it.mean()), 1), int(it.max())],
-            "max_true_rel_resid": float(rr.max())}
+            "max_true_rel_resid": float(rr.max()),
+            "graph_replay": bool(use_graph)}
 
 
 def cfg_next(H, h, log):
@@ -705,7 +708,7 @@ def run_suite(H, h, x, y, n, args, log, no_cpu=False, only=None):
     if "multidot" in want:
         res["multidot"] = cfg_multidot(H, h, log)
     if "batch" in want:
-        res["batch"] = cfg_batch(H, h, log)
+        res["batch"] = cfg_batch(H, h, log, use_graph=0 if args.batch_direct else 1)
     if "next" in want:
         res["next"] = cfg_next(H, h, log)
     return res
@@ -723,6 +726,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--op", default="dot2", choices=("dot2", "copy", "nrm2c", "exact"),
                     help="kernel timed for the headline (copy: the plain copy used to calibrate HBM counters)")
+    ap.add_argument("--batch-direct", action="store_true",
+                    help="batch suite with directly launched cycles instead of graph replays (profiling aid)")
     ap.add_argument("--suite", default=None,
                     help="comma-separated subset of the configurations after the headline: " + ",".join(SUITES))
     args = ap.parse_args()
@@ -764,6 +769,21 @@ def main():
         print(json.dumps({"op": args.op, "n": n, "ms": round(ms, 5), "gbs": round(nbytes / ms / 1e6, 2),
                           "algorithmic_bytes": int(nbytes)}), flush=True)
         return
+    # measured ceiling, same process, same vectors: the naive dot (same launch shape and
+    # single-launch reduction as Dot2, one fma per element: the read stream alone) and
+    # a copy; the naive dot first runs for WARM_S seconds so that the device is at its
+    # sustained clocks when the timed windows start (a cold device times its ramp)
+    log("warming the device for %.1f s with the naive dot" % WARM_S)
+    t_end = time.perf_counter() + WARM_S
+    nwarm = 0
+    while time.perf_counter() < t_end:
+        for _ in range(50):
+            L.hpb_dot_f64_async(h.ptr, n, x.ptr, y.ptr, dout.ptr)
+        h.sync()
+        nwarm += 50
+    ms_naive = timed(h, L.hpb_dot_f64_async, (h.ptr, n, x.ptr, y.ptr, dout.ptr), args.steps, args.warmup)
+    # in place (x -> x): the benchmark vectors keep their values
+    ms_copy = timed(h, L.hpb_copy_f64, (h.ptr, n, x.ptr, x.ptr), args.steps, args.warmup)
     log("timing %d Dot2 steps after %d warmup" % (args.steps, args.warmup))
     ms = timed(h, L.hpb_dot2_f64_async, (h.ptr, n, x.ptr, y.ptr, dout.ptr), args.steps, args.warmup)
     res = dout.numpy()
@@ -788,6 +808,12 @@ def main():
                                % (n.bit_length() - 1), "n": n,
                    "launch": {"block": cfg[0], "unroll": cfg[1], "grid": cfg[2], "vector_width": cfg[3]}},
         "hbm_peak_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "ceiling": {"dot_naive_gbs": round(16.0 * n / (ms_naive * 1e-3) / 1e9, 2),
+                    "copy_gbs": round(16.0 * n / (ms_copy * 1e-3) / 1e9, 2),
+                    "dot2_frac_of_dot_naive": round(ms_naive / ms, 4),
+                    "warmup": "naive dot for %.1f s (%d launches) before the timed windows" % (WARM_S, nwarm),
+                    "note": "naive dot: same launch shape and reduction as Dot2, one fma per element (the read "
+                            "stream); copy: reads and rewrites x in place (16 B per element)"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,

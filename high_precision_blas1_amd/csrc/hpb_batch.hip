@@ -224,10 +224,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ 
                 }
             }
             double s = (MODE == 1 && lane == 0) ? b[r] : 0.0, c = 0.0;
-            for (int k = rp[r] + lane; k < rp[r + 1]; k += LPR) {
-                const double a = MODE == 1 ? -val[k] : val[k];
-                dev::dot2_step<double>(s, c, a, x[col[k]]);
-            }
+            dev::csr_row_acc<double, LPR, true, MODE == 1>(rp[r], rp[r + 1], lane, col, val, x, s, c);
 #pragma unroll
             for (int m = LPR / 2; m >= 1; m >>= 1) {
                 double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);

@@ -42,12 +42,7 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
         const int end = rp[r + 1];
         T s = T(0), c = T(0);
         if (MODE == 1 && lane == 0) s = b[r];
-        for (int k = beg + lane; k < end; k += LPR) {
-            const T a = (MODE == 1) ? -val[k] : val[k];
-            const T xv = x[col[k]];
-            if (COMP) dev::dot2_step<T>(s, c, a, xv);
-            else s = dev::fma_rn<T>(a, xv, s);
-        }
+        dev::csr_row_acc<T, LPR, COMP, MODE == 1>(beg, end, lane, col, val, x, s, c);
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1) {
             T s2 = __shfl_xor(s, m, 64);

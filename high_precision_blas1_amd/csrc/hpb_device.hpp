@@ -182,6 +182,52 @@ __device__ __forceinline__ void store_nt<float4>(float4* p, const float4& v)
     __builtin_nontemporal_store(v.w, &p->w);
 }
 
+// One lane's share of a CSR row product: entries beg + lane, beg + lane + LPR, ...
+// accumulated into (s, c) in that order (Dot2 steps when COMP, fma otherwise; NEG
+// negates the values, for residual rows).  The lane's first two entries issue both
+// column/value loads and then both gathers of x before any arithmetic, so a row of
+// up to 2 LPR entries costs three dependent memory round trips (row pointer,
+// column/value, x) rather than two per entry; longer rows continue one entry at a
+// time.  The order of accumulation is that of the plain loop.
+template <typename T, int LPR, bool COMP, bool NEG>
+__device__ __forceinline__ void csr_row_acc(int beg, int end, int lane, const int* __restrict__ col,
+                                            const T* __restrict__ val, const T* __restrict__ x, T& s, T& c)
+{
+    const int k0 = beg + lane, k1 = k0 + LPR;
+    const bool v0 = k0 < end, v1 = k1 < end;
+    int c0 = 0, c1 = 0;
+    T a0 = T(0), a1 = T(0);
+    if (v0) {
+        c0 = col[k0];
+        a0 = val[k0];
+    }
+    if (v1) {
+        c1 = col[k1];
+        a1 = val[k1];
+    }
+    T x0 = T(0), x1 = T(0);
+    if (v0) x0 = x[c0];
+    if (v1) x1 = x[c1];
+    if (NEG) {
+        a0 = -a0;
+        a1 = -a1;
+    }
+    if (v0) {
+        if (COMP) dot2_step<T>(s, c, a0, x0);
+        else s = fma_rn<T>(a0, x0, s);
+    }
+    if (v1) {
+        if (COMP) dot2_step<T>(s, c, a1, x1);
+        else s = fma_rn<T>(a1, x1, s);
+    }
+    for (int k = k1 + LPR; k < end; k += LPR) {
+        const T a = NEG ? -val[k] : val[k];
+        const T xv = x[col[k]];
+        if (COMP) dot2_step<T>(s, c, a, xv);
+        else s = fma_rn<T>(a, xv, s);
+    }
+}
+
 // XCD-aware row ranges for the row kernels (SpMV, its fused forms).  Consecutive
 // workgroups are dispatched round-robin over the XCD_COUNT XCDs of the device, each
 // with its own L2; workgroup b runs on XCD b % XCD_COUNT.  Giving each XCD one

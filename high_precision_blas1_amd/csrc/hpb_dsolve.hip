@@ -756,11 +756,7 @@ __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict_
         const T rtr = (NQ >= 1 && lane == 0) ? rt[r] : T(0);   // issued with the row bounds
         T s = T(0), c = T(0);
         if (MODE == 1 && lane == 0) s = b[r];
-        for (int k = beg + lane; k < end; k += LPR) {
-            const T a = MODE == 1 ? -val[k] : val[k];
-            if (CROWS) dev::dot2_step<T>(s, c, a, x[col[k]]);
-            else s = dev::fma_rn<T>(a, x[col[k]], s);
-        }
+        dev::csr_row_acc<T, LPR, CROWS, MODE == 1>(beg, end, lane, col, val, x, s, c);
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1) {
             const T s2 = __shfl_xor(s, m, 64);

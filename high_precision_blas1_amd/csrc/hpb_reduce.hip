@@ -9,11 +9,12 @@
 //   * every lane keeps one (s, c) Ogita-Rump-Oishi accumulator; nothing is written
 //     per element (the reference zeroes and re-reads an N-sized error array);
 //   * wave64 butterfly merge with TwoSum, LDS across the waves of a block;
-//   * per-block (s, c) partials go to a handle-owned workspace of 2G words; the
-//     block that takes the last ticket merges them in block-index order, so the
-//     result does not depend on which block finishes last (reproducible); no
-//     floating-point atomics, no in-place multi-pass (the reference's passes read
-//     and write the same array from different blocks, impl.cuh:1247-1253);
+//   * per-block (s, c) partials go to a handle-owned workspace of 2G words; a
+//     one-block kernel queued behind the reduction merges them in block-index
+//     order, so the result does not depend on the order in which the blocks ran
+//     (reproducible); no floating-point atomics, no in-place multi-pass (the
+//     reference's passes read and write the same array from different blocks,
+//     impl.cuh:1247-1253);
 //   * the norms are safe against overflow and underflow, as the reference's
 //     standard-precision norm (cuBLAS nrm2, gpu_vector_operations.h:315-327) is:
 //     the launch also takes M = max |x_i|; when M lies outside [lo, hi] (2^-400,
@@ -160,16 +161,16 @@ __device__ __forceinline__ void nrm_rescaled_out(T* out, T s, T c, int k, bool i
     fix->need = 0;
 }
 
-// Fixed-order final stage, run by the block holding the last ticket: merges the
-// G partials in index order (thread t takes t, t+BLOCK, ...; then block_reduce).
+// Fixed-order final stage: merges the G partials in index order (thread t takes
+// t, t+BLOCK, ...; then block_reduce).
 template <typename T, bool COMP, int BLOCK>
 __device__ __forceinline__ void final_merge(const T* part, unsigned G, T& fs, T& fc)
 {
     fs = T(0);
     fc = T(0);
     for (unsigned i = threadIdx.x; i < G; i += BLOCK) {
-        T ps = ld_agent<T>(&part[2 * i]);
-        T pc = ld_agent<T>(&part[2 * i + 1]);
+        T ps = part[2 * i];
+        T pc = part[2 * i + 1];
         if (COMP) dev::merge<T>(fs, fc, ps, pc);
         else fs = fs + ps;
     }
@@ -196,24 +197,21 @@ __device__ __forceinline__ bool take_ticket(unsigned* ticket, unsigned G)
 
 // NT: stream the full tiles with nontemporal loads (no reuse is expected).
 // RESCALE (norms only): the rescaling launch.  It does nothing unless the norm
-// launch before it asked for it (fix->need), then sums over x_i 2^k.
+// launch before it asked for it (fix->need), then sums over x_i 2^k; its merge
+// kernel clears the request.
 template <typename T, int OP, int BLOCK, int UNROLL, bool VEC, bool NT = false, bool RESCALE = false>
 __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict__ x, const T* __restrict__ y,
-                                                  T* __restrict__ part, unsigned* __restrict__ ticket,
-                                                  T* __restrict__ out, const int* __restrict__ skip,
-                                                  nrm_fix* __restrict__ fix)
+                                                  T* __restrict__ part, T* __restrict__ out,
+                                                  const int* __restrict__ skip, nrm_fix* __restrict__ fix)
 {
     using tr = red_traits<OP>;
     static_assert(!RESCALE || tr::nrm, "only the norms are rescaled");
     constexpr bool TRACK = tr::nrm && !RESCALE;   // take M = max |x_i| on the way
     // solver graphs pass a device flag: once it is set the launch does nothing
-    // (every block returns, so the ticket is untouched)
     if (skip != nullptr && *skip) return;
     int k = 0;
     bool inf_mode = false;
     if constexpr (RESCALE) {
-        // read by every block before any block can take the last ticket, so the
-        // clearing store of the last block cannot overtake it
         if (fix->need == 0) return;
         const double am = fix->amax;
         if (am > 1.7976931348623157e308) {
@@ -294,24 +292,51 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict_
         }
         return;
     }
+    // G > 1: the per-block partials; k_reduce_fin, launched next on the same stream,
+    // merges them (the kernel boundary makes them visible to it)
     if (tid == 0) {
-        st_agent<T>(&part[2 * b], s);
-        st_agent<T>(&part[2 * b + 1], c);
-        if constexpr (TRACK) st_agent<T>(&part[2 * G + b], m);
+        part[2 * b] = s;
+        part[2 * b + 1] = c;
+        if constexpr (TRACK) part[2 * G + b] = m;
     }
-    if (!take_ticket(ticket, G)) return;
+}
+
+// Fixed-order final stage of a G > 1 reduction: one block of the reduction's BLOCK
+// threads merges the G partials in block-index order (thread t takes t, t + BLOCK,
+// ...; then the block reduction), so the result does not depend on the order in
+// which the blocks of k_reduce ran.  A separate launch instead of a last-block
+// ticket: on MI355X the agent-scope release each block's ticket needs cost about
+// 7 us per 300 us Dot2 at N = 2^27, the second launch about 2 us.
+template <typename T, int OP, int BLOCK, bool RESCALE = false>
+__global__ __launch_bounds__(BLOCK) void k_reduce_fin(unsigned G, const T* __restrict__ part, T* __restrict__ out,
+                                                      const int* __restrict__ skip, nrm_fix* __restrict__ fix)
+{
+    using tr = red_traits<OP>;
+    constexpr bool TRACK = tr::nrm && !RESCALE;
+    if (skip != nullptr && *skip) return;
+    int k = 0;
+    bool inf_mode = false;
+    if constexpr (RESCALE) {
+        if (fix->need == 0) return;
+        const double am = fix->amax;
+        if (am > 1.7976931348623157e308) {
+            inf_mode = true;
+        } else {
+            (void)__builtin_frexp(am, &k);
+            k = -k;
+        }
+    }
     T fs, fc;
     final_merge<T, tr::comp, BLOCK>(part, G, fs, fc);
     T fm = T(0);
     if constexpr (TRACK) {
-        for (unsigned i = tid; i < G; i += BLOCK) fm = maxv<T>(fm, ld_agent<T>(&part[2 * G + i]));
+        for (unsigned i = threadIdx.x; i < G; i += BLOCK) fm = maxv<T>(fm, part[2 * G + i]);
         fm = block_max<T, BLOCK>(fm);
     }
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
         if constexpr (TRACK) nrm_finish<T, OP>(out, fs, fc, fm, fix);
         else if constexpr (RESCALE) nrm_rescaled_out<T>(out, fs, fc, k, inf_mode, fix);
         else write_out<T, OP>(out, fs, fc);
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -319,8 +344,9 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(size_t n, const T* __restrict_
 // min and max (gpu_reduction::min_max, source/common/gpu_reduction.h:61-68,
 // gpu_reduction_impl.cuh:82-264): NaN entries are skipped, as the reference's
 // comparisons skip them; -0 orders below +0, so the result does not depend on the
-// order of the merges.  Same launch shape, partials and last-block ticket as the
-// one-input reductions; out2 = { min, max }, { +inf, -inf } for an empty x.
+// order of the merges.  Same tile walk and partials as the one-input reductions;
+// the block that takes the last ticket merges the partials (a blocking call, whose
+// copy back outweighs the ticket); out2 = { min, max }, { +inf, -inf } for an empty x.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T min_ord(T a, T b)
@@ -464,33 +490,36 @@ static red_shape reduce_shape(const hpb_handle_s* h, size_t n, bool aligned, boo
 }
 
 template <typename T, int OP, int BLOCK, int UNROLL, bool RS>
-static void launch_bu(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
-                      unsigned* ticket, T* out, const int* skip, nrm_fix* fix)
+static void launch_bu(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part, T* out,
+                      const int* skip, nrm_fix* fix)
 {
     if (sh.vecw > 1 && (sh.flags & HPB_TUNE_NONTEMPORAL) && UNROLL >= 2)
         hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true, true, RS>), dim3(sh.grid), dim3(BLOCK), 0, st, n, x,
-                           y, part, ticket, out, skip, fix);
+                           y, part, out, skip, fix);
     else if (sh.vecw > 1)
         hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, true, false, RS>), dim3(sh.grid), dim3(BLOCK), 0, st, n,
-                           x, y, part, ticket, out, skip, fix);
+                           x, y, part, out, skip, fix);
     else
         hipLaunchKernelGGL((k_reduce<T, OP, BLOCK, UNROLL, false, false, RS>), dim3(sh.grid), dim3(BLOCK), 0, st, n,
-                           x, y, part, ticket, out, skip, fix);
+                           x, y, part, out, skip, fix);
+    if (sh.grid > 1)
+        hipLaunchKernelGGL((k_reduce_fin<T, OP, BLOCK, RS>), dim3(1), dim3(BLOCK), 0, st, (unsigned)sh.grid,
+                           (const T*)part, out, skip, fix);
 }
 
 template <typename T, int OP, bool RS = false>
-static void launch_red(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part,
-                       unsigned* ticket, T* out, const int* skip, nrm_fix* fix)
+static void launch_red(const red_shape& sh, hipStream_t st, size_t n, const T* x, const T* y, T* part, T* out,
+                       const int* skip, nrm_fix* fix)
 {
     switch (sh.block * 16 + sh.unroll) {
-    case 256 * 16 + 1: launch_bu<T, OP, 256, 1, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
-    case 256 * 16 + 2: launch_bu<T, OP, 256, 2, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
-    case 256 * 16 + 8: launch_bu<T, OP, 256, 8, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
-    case 512 * 16 + 1: launch_bu<T, OP, 512, 1, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
-    case 512 * 16 + 2: launch_bu<T, OP, 512, 2, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
-    case 512 * 16 + 4: launch_bu<T, OP, 512, 4, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
-    case 512 * 16 + 8: launch_bu<T, OP, 512, 8, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
-    default: launch_bu<T, OP, 256, 4, RS>(sh, st, n, x, y, part, ticket, out, skip, fix); break;
+    case 256 * 16 + 1: launch_bu<T, OP, 256, 1, RS>(sh, st, n, x, y, part, out, skip, fix); break;
+    case 256 * 16 + 2: launch_bu<T, OP, 256, 2, RS>(sh, st, n, x, y, part, out, skip, fix); break;
+    case 256 * 16 + 8: launch_bu<T, OP, 256, 8, RS>(sh, st, n, x, y, part, out, skip, fix); break;
+    case 512 * 16 + 1: launch_bu<T, OP, 512, 1, RS>(sh, st, n, x, y, part, out, skip, fix); break;
+    case 512 * 16 + 2: launch_bu<T, OP, 512, 2, RS>(sh, st, n, x, y, part, out, skip, fix); break;
+    case 512 * 16 + 4: launch_bu<T, OP, 512, 4, RS>(sh, st, n, x, y, part, out, skip, fix); break;
+    case 512 * 16 + 8: launch_bu<T, OP, 512, 8, RS>(sh, st, n, x, y, part, out, skip, fix); break;
+    default: launch_bu<T, OP, 256, 4, RS>(sh, st, n, x, y, part, out, skip, fix); break;
     }
 }
 
@@ -512,13 +541,13 @@ static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev
     T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
     T* part = reinterpret_cast<T*>(h->part);
     nrm_fix* fix = reinterpret_cast<nrm_fix*>(h->nrmfix);
-    launch_red<T, OP>(sh, h->stream, n, x, y, part, h->ticket, out, skip, fix);
+    launch_red<T, OP>(sh, h->stream, n, x, y, part, out, skip, fix);
     HPB_LAUNCH_CHECK();
     if constexpr (red_traits<OP>::nrm) {
         // asynchronous: the rescaling launch is always queued (it returns at once
         // unless the norm launch asked for it); blocking: only when asked
         if (dev_out) {
-            launch_red<T, OP, true>(sh, h->stream, n, x, y, part, h->ticket, out, skip, fix);
+            launch_red<T, OP, true>(sh, h->stream, n, x, y, part, out, skip, fix);
             HPB_LAUNCH_CHECK();
             return HPB_OK;
         }
@@ -528,7 +557,7 @@ static int reduce_entry(hpb_handle_t h, size_t n, const T* x, const T* y, T* dev
         HPB_HIP(hipStreamSynchronize(h->stream));
         const nrm_fix* hf = reinterpret_cast<const nrm_fix*>(reinterpret_cast<const char*>(h->hres) + 64);
         if (hf->need) {
-            launch_red<T, OP, true>(sh, h->stream, n, x, y, part, h->ticket, out, skip, fix);
+            launch_red<T, OP, true>(sh, h->stream, n, x, y, part, out, skip, fix);
             HPB_LAUNCH_CHECK();
             HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T), hipMemcpyDeviceToHost, h->stream));
             HPB_HIP(hipStreamSynchronize(h->stream));
