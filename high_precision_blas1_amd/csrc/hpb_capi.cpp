@@ -92,6 +92,10 @@ int hpb_create(hpb_handle_t* out, void* stream)
         if ((e = hipEventCreate(&h->ev[i])) != hipSuccess) return fail(e, __LINE__, "hipEventCreate");
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, __LINE__, "hipDeviceSynchronize");
     default_tuning(h);
+    if (int rc = hpb::noalloc_init()) {
+        hpb_destroy(h);
+        return rc;
+    }
     *out = h;
     return HPB_OK;
 }

@@ -54,6 +54,9 @@ struct nrm_fix {
 };
 
 void set_error(const char* file, int line, const std::string& msg);
+// looks up the current device's address of the no-allocation scratch array
+// (hpb_multidot_f64_noalloc); called by hpb_create
+int noalloc_init();
 int hip_status(hipError_t e, const char* file, int line, const char* what);
 
 // Asynchronous launchers used by the device-resident solvers.  Results stay in

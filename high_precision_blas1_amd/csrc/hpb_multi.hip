@@ -16,6 +16,8 @@
 // launch for grids of more than a few hundred blocks.)
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <mutex>
 #include <type_traits>
 
 #include "hpb_device.hpp"
@@ -188,14 +190,16 @@ __global__ __launch_bounds__(256) void k_multiaxpy(size_t n, int k, coef_pack<T>
 
 static inline bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+// Checks the arguments and launches the multi-dot and its merge on stream st with
+// launch shape tu on a device of ncu CUs; part holds the per-block partials, at
+// most max_pairs (s, c) pairs (the grid is capped so that g k <= max_pairs).
 template <typename T>
-static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, const T* w, int comp, T* dev_out,
-                          T* host_out, const int* skip = nullptr)
+static int multidot_launch(hipStream_t st, int ncu, const hpb_tuning_t& tu, int k, const T* const* V, size_t n,
+                           const T* w, int comp, T* part, size_t max_pairs, T* out, const int* skip)
 {
-    HPB_CLEAR_STALE();
-    HPB_REQUIRE(h && V && (n == 0 || w), HPB_ERR_INVALID, "hpb_multidot: null argument");
+    HPB_REQUIRE(V && (n == 0 || w), HPB_ERR_INVALID, "hpb_multidot: null argument");
     HPB_REQUIRE(k >= 1 && k <= HPB_MAX_MULTIDOT, HPB_ERR_INVALID, "hpb_multidot: need 1 <= k <= 64");
-    HPB_REQUIRE(dev_out || host_out, HPB_ERR_INVALID, "hpb_multidot: null output");
+    HPB_REQUIRE(out, HPB_ERR_INVALID, "hpb_multidot: null output");
     ptr_pack<T> pk;
     bool vec = al16(w);
     for (int j = 0; j < k; ++j) {
@@ -204,26 +208,23 @@ static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, co
         vec = vec && al16(V[j]);
     }
     for (int j = k; j < HPB_MAX_MULTIDOT; ++j) pk.p[j] = nullptr;
-    // launch shape from the handle's tuning (kind 3): columns per chunk (4 or 8),
-    // blocks over all chunks, nontemporal loads
-    const hpb_tuning_t& tu = h->tune_multi[sizeof(T) == 8 ? 0 : 1];
+    // launch shape (tuning kind 3): columns per chunk (4 or 8), blocks over all
+    // chunks, nontemporal loads
     const int KCr = tu.unroll == 4 ? 4 : 8;
     const bool nt = (tu.flags & HPB_TUNE_NONTEMPORAL) != 0;
     const int nch = (k + KCr - 1) / KCr;
     const size_t W = vec ? dev::vec16<T>::W : 1;
     const size_t nv = n / W;
     size_t g = (nv + 255) / 256;
-    const size_t cus = (size_t)(h->num_cu > 0 ? h->num_cu : 256);
+    const size_t cus = (size_t)(ncu > 0 ? ncu : 256);
     size_t cap = (size_t)tu.grid_max / (size_t)nch;
     if (cap < cus) cap = cus;   // at least one block per CU per chunk
-    if (cap * (size_t)k > (size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) cap = ((size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT) / k;
+    if (cap * (size_t)k > max_pairs) cap = max_pairs / (size_t)k;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
-    T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
-    T* part = reinterpret_cast<T*>(h->part);
     dim3 grid((unsigned)g, (unsigned)nch);
 #define HPB_MDOT(KC_, C_, V_, NT_)                                                                          \
-    hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_>), grid, dim3(256), 0, h->stream, n, k, pk, w, part, skip)
+    hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_>), grid, dim3(256), 0, st, n, k, pk, w, part, skip)
 #define HPB_MDOT_KC(C_, V_, NT_)                  \
     do {                                          \
         if (KCr == 4) HPB_MDOT(4, C_, V_, NT_);   \
@@ -232,15 +233,30 @@ static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, co
     if (comp) {
         if (vec) { if (nt) HPB_MDOT_KC(true, true, true); else HPB_MDOT_KC(true, true, false); }
         else HPB_MDOT_KC(true, false, false);
-        hipLaunchKernelGGL((k_multidot_fin<T, true>), dim3(1), dim3(1024), 0, h->stream, part, (unsigned)g, k, out, skip);
+        hipLaunchKernelGGL((k_multidot_fin<T, true>), dim3(1), dim3(1024), 0, st, part, (unsigned)g, k, out, skip);
     } else {
         if (vec) { if (nt) HPB_MDOT_KC(false, true, true); else HPB_MDOT_KC(false, true, false); }
         else HPB_MDOT_KC(false, false, false);
-        hipLaunchKernelGGL((k_multidot_fin<T, false>), dim3(1), dim3(1024), 0, h->stream, part, (unsigned)g, k, out, skip);
+        hipLaunchKernelGGL((k_multidot_fin<T, false>), dim3(1), dim3(1024), 0, st, part, (unsigned)g, k, out, skip);
     }
 #undef HPB_MDOT_KC
 #undef HPB_MDOT
     HPB_LAUNCH_CHECK();
+    return HPB_OK;
+}
+
+template <typename T>
+static int multidot_entry(hpb_handle_t h, int k, const T* const* V, size_t n, const T* w, int comp, T* dev_out,
+                          T* host_out, const int* skip = nullptr)
+{
+    HPB_CLEAR_STALE();
+    HPB_REQUIRE(h, HPB_ERR_INVALID, "hpb_multidot: null handle");
+    HPB_REQUIRE(dev_out || host_out, HPB_ERR_INVALID, "hpb_multidot: null output");
+    T* out = dev_out ? dev_out : reinterpret_cast<T*>(h->dres);
+    const int rc = multidot_launch<T>(h->stream, h->num_cu, h->tune_multi[sizeof(T) == 8 ? 0 : 1], k, V, n, w, comp,
+                                      reinterpret_cast<T*>(h->part), (size_t)HPB_MAX_GRID * HPB_MAX_MULTIDOT, out,
+                                      skip);
+    if (rc != HPB_OK) return rc;
     if (!dev_out) {
         HPB_HIP(hipMemcpyAsync(h->hres, h->dres, 4 * sizeof(T) * k, hipMemcpyDeviceToHost, h->stream));
         HPB_HIP(hipStreamSynchronize(h->stream));
@@ -310,11 +326,80 @@ template int multiaxpy_devcoef<double>(hpb_handle_t, int, const double*, const d
 template int multiaxpy_devcoef<float>(hpb_handle_t, int, const float*, const float* const*, size_t, float*, bool,
                                       const int*);
 
+// ---------------------------------------------------------------------------
+// No-allocation form (hpb_multidot_f64_noalloc): the per-block partials go to the
+// file-scope device array `scratch` instead of a handle's workspace, for callers
+// that cannot allocate (graph capture, serving loops).  Each device holds its own
+// copy of the array, so its address is looked up once per device with that device
+// current (hipGetSymbolAddress) and kept; host code uses only that address.  All
+// calls on one device share the array: calls that overlap on one device race.
+// Every launch overwrites the partials it reads, so nothing needs clearing.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_PAIRS = 16384;   // (s, c) pairs: 256 KiB per device
+constexpr int NOALLOC_DEVICES = 64;
+static __device__ __align__(256) double scratch[2 * NOALLOC_PAIRS];
+
+struct noalloc_dev {
+    std::atomic<int> ready{0};
+    double* ptr = nullptr;
+    int ncu = 0;
+};
+static noalloc_dev g_noalloc[NOALLOC_DEVICES];
+static std::mutex g_noalloc_mu;
+
+// the current device's scratch address and CU count (looked up on its first use)
+static int noalloc_scratch(double** ptr, int* ncu)
+{
+    int dev = 0;
+    HPB_HIP(hipGetDevice(&dev));
+    HPB_REQUIRE(dev >= 0 && dev < NOALLOC_DEVICES, HPB_ERR_UNSUPPORTED, "hpb_multidot_noalloc: device index >= 64");
+    noalloc_dev& d = g_noalloc[dev];
+    if (!d.ready.load(std::memory_order_acquire)) {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        if (!d.ready.load(std::memory_order_relaxed)) {
+            void* p = nullptr;
+            HPB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            int cu = 0;
+            HPB_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+            d.ptr = static_cast<double*>(p);
+            d.ncu = cu;
+            d.ready.store(1, std::memory_order_release);
+        }
+    }
+    *ptr = d.ptr;
+    *ncu = d.ncu;
+    return HPB_OK;
+}
+
+int noalloc_init()
+{
+    double* p = nullptr;
+    int cu = 0;
+    return noalloc_scratch(&p, &cu);
+}
+
 }  // namespace hpb
 
 using namespace hpb;
 
 extern "C" {
+
+int hpb_multidot_f64_noalloc(void* stream, int k, const double* const* V, size_t n, const double* w,
+                             int compensated, double* dev_out4k)
+{
+    HPB_CLEAR_STALE();
+    HPB_REQUIRE(k >= 1 && k <= HPB_MAX_MULTIDOT, HPB_ERR_INVALID,
+                "hpb_multidot_f64_noalloc: need 1 <= k <= 64 (the scratch array holds the partials of 64 columns)");
+    HPB_REQUIRE(dev_out4k, HPB_ERR_INVALID, "hpb_multidot_f64_noalloc: null device output");
+    double* part = nullptr;
+    int ncu = 0;
+    const int rc = noalloc_scratch(&part, &ncu);
+    if (rc != HPB_OK) return rc;
+    // the launch shape of a new handle's default tuning (hpb_create)
+    const hpb_tuning_t tu = {256, 8, 8 * (ncu > 0 ? ncu : 256), HPB_TUNE_NONTEMPORAL};
+    return multidot_launch<double>((hipStream_t)stream, ncu, tu, k, V, n, w, compensated, part, NOALLOC_PAIRS,
+                                   dev_out4k, nullptr);
+}
 
 int hpb_multidot_f64(hpb_handle_t h, int k, const double* const* V, size_t n, const double* w, int compensated,
                      double* out4k)

@@ -252,6 +252,19 @@ int hpb_multidot_f64_async(hpb_handle_t h, int k, const double* const* V, size_t
                            double* dev_out4k);
 int hpb_multidot_f32_async(hpb_handle_t h, int k, const float* const* V, size_t n, const float* w, int compensated,
                            float* dev_out4k);
+/* No counterpart in the reference: hpb_multidot_f64_async for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No handle: the work goes on `stream` (a hipStream_t, may be
+ * null for the default stream), with the launch shape of a new handle's default
+ * tuning, and the per-block partials go to a file-scope device array of the library
+ * (256 KiB per device, 16384 (s, c) pairs, looked up once per device: at hpb_create,
+ * or at the first call on that device) instead of a handle's workspace; nothing is
+ * allocated.  Same results as hpb_multidot_f64_async with default tuning on a device
+ * where the array's cap does not bind (any k on MI355X).  Rejects k outside 1..64
+ * (HPB_ERR_INVALID).  All calls on one device share the array: calls that overlap
+ * on one device race. */
+int hpb_multidot_f64_noalloc(void* stream, int k, const double* const* V, size_t n, const double* w,
+                             int compensated, double* dev_out4k);
 /* y := y + sum_{j<k} a_j V_j in one pass (k <= 64); compensated evaluates every
  * element as a Dot2 over its k + 1 terms.  a is a host array (the _dev form
  * reads the k coefficients from device memory). */

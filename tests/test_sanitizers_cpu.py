@@ -6,8 +6,9 @@
   -fno-sanitize-recover, must run clean and print the same results as the plain
   build.
 * The library's MatrixMarket reader and writer (csrc/hpb_mm.cpp, with the
-  library's status plumbing csrc/hpb_capi.cpp) compiled with the sanitizers on
-  the host side only (hipcc -Xarch_host), driven through good and malformed files.
+  library's status plumbing csrc/hpb_capi.cpp and what it links to) compiled with
+  the sanitizers on the host side only (hipcc -Xarch_host), driven through good
+  and malformed files.
 No GPU code is instrumented and nothing here needs a GPU.
 """
 import os
@@ -51,7 +52,8 @@ def test_matrix_market_sanitized(tmp_path):
     if not os.path.exists(hipcc):
         pytest.skip("hipcc not found")
     objs = []
-    for src in ("hpb_mm.cpp", "hpb_capi.cpp"):
+    # hpb_capi.cpp's hpb_create refers to the no-allocation set-up in hpb_multi.hip
+    for src in ("hpb_mm.cpp", "hpb_capi.cpp", "hpb_multi.hip"):
         obj = str(tmp_path / (src + ".o"))
         # host side instrumented only; the device pass of these files is empty
         subprocess.check_call([hipcc, "-x", "hip", "-O1", "-g", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
