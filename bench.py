@@ -536,11 +536,16 @@ def cfg_multidot(H, h, log):
     return {"n": nmax, "rows": rows, "length_sweep": sweep}
 
 
+BATCH_LAUNCH = {"graph": 1, "direct": 0, "persistent": 2}
+
+
 def cfg_batch(H, h, log, nsys=512, recalc=0, use_graph=1):
     """512 independent 2-D 5-point diffusion systems, sides uniform in 64..1024 (numpy
     default_rng(3)), log-normal coefficients (sigma 1, seed 3), x* uniform in [-1, 1]
     (seed 7), BiCGStab(2) with Jacobi right preconditioning to 1e-10 true relative
-    residual per system, one captured hipGraph per cycle over all live systems.
+    residual per system; use_graph 1: one captured hipGraph per cycle over all live
+    systems, 0: the same kernels launched directly, 2: every cycle in one persistent
+    cooperative launch.
     recalc: the true residual replaces the recursive one every recalc cycles (0:
     never; convergence is confirmed on the true residual either way)."""
     import numpy as np
@@ -576,7 +581,7 @@ def cfg_batch(H, h, log, nsys=512, recalc=0, use_graph=1):
             "solve_s": round(secs.value, 3), "cycles": cyc.value, "converged": int(cv.sum()),
             "iters_min_mean_max": [int(it.min()), round(float(it.mean()), 1), int(it.max())],
             "max_true_rel_resid": float(rr.max()),
-            "graph_replay": bool(use_graph)}
+            "launch": {v: k for k, v in BATCH_LAUNCH.items()}[use_graph]}
 
 
 def cfg_next(H, h, log):
@@ -708,7 +713,8 @@ def run_suite(H, h, x, y, n, args, log, no_cpu=False, only=None):
     if "multidot" in want:
         res["multidot"] = cfg_multidot(H, h, log)
     if "batch" in want:
-        res["batch"] = cfg_batch(H, h, log, use_graph=0 if args.batch_direct else 1)
+        mode = "direct" if args.batch_direct else args.batch_launch
+        res["batch"] = cfg_batch(H, h, log, use_graph=BATCH_LAUNCH[mode])
     if "next" in want:
         res["next"] = cfg_next(H, h, log)
     return res
@@ -728,6 +734,9 @@ def main():
                     help="kernel timed for the headline (copy: the plain copy used to calibrate HBM counters)")
     ap.add_argument("--batch-direct", action="store_true",
                     help="batch suite with directly launched cycles instead of graph replays (profiling aid)")
+    ap.add_argument("--batch-launch", default="graph", choices=tuple(BATCH_LAUNCH),
+                    help="how the batch suite runs its cycles: graph replays, direct launches, or one persistent "
+                         "cooperative launch")
     ap.add_argument("--suite", default=None,
                     help="comma-separated subset of the configurations after the headline: " + ",".join(SUITES))
     args = ap.parse_args()

@@ -94,6 +94,7 @@ struct batch_t {
     hipGraph_t graph2 = nullptr;    // cycle on the recursive residual
     hipGraphExec_t exec2 = nullptr;
     int verbose = 0;                // print the active count at every check
+    struct pbar* bar = nullptr;     // grid barrier state of the persistent launch
 };
 enum { VB = 0, VX, VR0, VR1, VR2, VU0, VU1, VU2, VRT, VXS };
 
@@ -182,14 +183,14 @@ __global__ __launch_bounds__(256) void k_batch_build(int nsys, int64_t nrows, co
 //          (r1, r1), (r2, r1), (r2, r2), (r0, r1), (r0, r2) in slots 0..4.
 // With only != null the tiles of systems with only[sys] == 0 are skipped.
 template <int MODE, int EPI>
-__global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ tiles, const int* __restrict__ list,
-                                                      const int* __restrict__ count, const int* __restrict__ active,
-                                                      const int* __restrict__ rp, const int* __restrict__ col,
-                                                      const double* __restrict__ val, const double* __restrict__ x,
-                                                      const double* __restrict__ b, double* __restrict__ y,
-                                                      const double* __restrict__ rt,
-                                                      const double* __restrict__ z, double* __restrict__ part,
-                                                      const int* __restrict__ only)
+__device__ __forceinline__ void bspmv_body(unsigned bid, unsigned nblk, const btile* __restrict__ tiles,
+                                           const int* __restrict__ list, const int* __restrict__ count,
+                                           const int* __restrict__ active, const int* __restrict__ rp,
+                                           const int* __restrict__ col, const double* __restrict__ val,
+                                           const double* __restrict__ x, const double* __restrict__ b,
+                                           double* __restrict__ y, const double* __restrict__ rt,
+                                           const double* __restrict__ z, double* __restrict__ part,
+                                           const int* __restrict__ only)
 {
     static_assert(EPI == 0 || EPI == 1 || EPI == 2 || EPI == 5, "k_bspmv: epilogue");
     constexpr int NQ = EPI == 0 ? 1 : EPI;
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ 
     __shared__ double wtot[2][NQ][NW][2];
     int buf = 0;
     const int nt = *count;
-    const dev::xcd_rows Q = dev::xcd_rows::of(nt, 1);
+    const dev::xcd_rows Q = dev::xcd_rows::of(nt, 1, nblk, bid);
     for (long q = Q.first; q < Q.end; q += Q.stride) {
         const int ti = list[q];
         const btile t = tiles[ti];
@@ -271,6 +272,19 @@ __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ 
     }
 }
 
+template <int MODE, int EPI>
+__global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ tiles, const int* __restrict__ list,
+                                                      const int* __restrict__ count, const int* __restrict__ active,
+                                                      const int* __restrict__ rp, const int* __restrict__ col,
+                                                      const double* __restrict__ val, const double* __restrict__ x,
+                                                      const double* __restrict__ b, double* __restrict__ y,
+                                                      const double* __restrict__ rt,
+                                                      const double* __restrict__ z, double* __restrict__ part,
+                                                      const int* __restrict__ only)
+{
+    bspmv_body<MODE, EPI>(blockIdx.x, gridDim.x, tiles, list, count, active, rp, col, val, x, b, y, rt, z, part, only);
+}
+
 struct dot_pairs {
     const double* a[BQ];
     const double* b[BQ];
@@ -325,14 +339,43 @@ __device__ __forceinline__ void merge_sys(int s, const int* tile_first, const do
 // bnorm); 5 recursive-residual check: a system that meets the tolerance is marked
 // for confirmation; 6 true-residual check of the marked systems.  The NQ dot
 // products come from slots Q0 ..
+// One system per wave: wave w of block bid takes systems (bid * waves + w) +
+// k * nblk * waves.  The 256-thread blocks of the standalone kernel take one
+// system per wave.
+template <int NQ, int PHASE, int Q0>
+__device__ __forceinline__ void bsys_one(int s, const int* __restrict__ tile_first, const double* __restrict__ part,
+                                         double* __restrict__ st, int* __restrict__ active, int* __restrict__ iters,
+                                         int* __restrict__ nactive, int* __restrict__ confirm, double tol2, int first);
+
+template <int NQ, int PHASE, int Q0 = 0>
+__device__ __forceinline__ void bsys_body(unsigned bid, unsigned nblk, int nsys, const int* __restrict__ tile_first,
+                                          const double* __restrict__ part, double* __restrict__ st,
+                                          int* __restrict__ active, int* __restrict__ iters,
+                                          int* __restrict__ nactive, int* __restrict__ confirm, double tol2,
+                                          int first)
+{
+    const int waves = (int)(blockDim.x >> 6);
+    for (int s = (int)bid * waves + (int)(threadIdx.x >> 6); s < nsys; s += (int)nblk * waves)
+        bsys_one<NQ, PHASE, Q0>(s, tile_first, part, st, active, iters, nactive, confirm, tol2, first);
+}
+
 template <int NQ, int PHASE, int Q0 = 0>
 __global__ __launch_bounds__(256) void k_bsys(int nsys, const int* __restrict__ tile_first,
                                               const double* __restrict__ part, double* __restrict__ st,
                                               int* __restrict__ active, int* __restrict__ iters, int* __restrict__ nactive,
                                               int* __restrict__ confirm, double tol2, int first)
 {
-    const int s = (blockIdx.x * 256 + threadIdx.x) >> 6;
-    if (s >= nsys || !active[s]) return;
+    bsys_body<NQ, PHASE, Q0>(blockIdx.x, gridDim.x, nsys, tile_first, part, st, active, iters, nactive, confirm, tol2,
+                             first);
+}
+
+// the work of one wave on system s (every lane of the wave calls it)
+template <int NQ, int PHASE, int Q0>
+__device__ __forceinline__ void bsys_one(int s, const int* __restrict__ tile_first, const double* __restrict__ part,
+                                         double* __restrict__ st, int* __restrict__ active, int* __restrict__ iters,
+                                         int* __restrict__ nactive, int* __restrict__ confirm, double tol2, int first)
+{
+    if (!active[s]) return;
     if (PHASE == 6 && !confirm[s]) return;
     static_assert(Q0 + NQ <= BQ, "k_bsys: slots");
     double d[NQ];
@@ -517,13 +560,13 @@ __device__ __forceinline__ void upd_step(int64_t r, int64_t end, const upd_coef&
 // on the recursive residual): per-tile partials of (r0, r0) and (r0, r~) of the new
 // r0 in slots 0 and 1, as the true-residual SpMV leaves them.
 template <int KIND, bool DOTS = false>
-__global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ tiles, const int* __restrict__ list,
-                                                     const int* __restrict__ count, const int* __restrict__ active,
-                                                     const double* __restrict__ st, double* __restrict__ x,
-                                                     double* __restrict__ r0, double* __restrict__ r1,
-                                                     double* __restrict__ r2, double* __restrict__ u0,
-                                                     double* __restrict__ u1, double* __restrict__ u2,
-                                                     const double* __restrict__ rt, double* __restrict__ part)
+__device__ __forceinline__ void bupd_body(unsigned bid, unsigned nblk, const btile* __restrict__ tiles,
+                                          const int* __restrict__ list, const int* __restrict__ count,
+                                          const int* __restrict__ active, const double* __restrict__ st,
+                                          double* __restrict__ x, double* __restrict__ r0, double* __restrict__ r1,
+                                          double* __restrict__ r2, double* __restrict__ u0, double* __restrict__ u1,
+                                          double* __restrict__ u2, const double* __restrict__ rt,
+                                          double* __restrict__ part)
 {
     static_assert(!DOTS || KIND == 4, "k_bupd: dot products of the MR update only");
     constexpr int NW = BT_THREADS / 64;
@@ -531,7 +574,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ t
     __shared__ double wtot[2][2][NW][2];
     int buf = 0;
     const int nt = *count;
-    const dev::xcd_rows Q = dev::xcd_rows::of(nt, 1);
+    const dev::xcd_rows Q = dev::xcd_rows::of(nt, 1, nblk, bid);
     for (long q = Q.first; q < Q.end; q += Q.stride) {
         const int ti = list[q];
         const btile t = tiles[ti];
@@ -564,11 +607,24 @@ __global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ t
     }
 }
 
+template <int KIND, bool DOTS = false>
+__global__ __launch_bounds__(BT_THREADS) void k_bupd(const btile* __restrict__ tiles, const int* __restrict__ list,
+                                                     const int* __restrict__ count, const int* __restrict__ active,
+                                                     const double* __restrict__ st, double* __restrict__ x,
+                                                     double* __restrict__ r0, double* __restrict__ r1,
+                                                     double* __restrict__ r2, double* __restrict__ u0,
+                                                     double* __restrict__ u1, double* __restrict__ u2,
+                                                     const double* __restrict__ rt, double* __restrict__ part)
+{
+    bupd_body<KIND, DOTS>(blockIdx.x, gridDim.x, tiles, list, count, active, st, x, r0, r1, r2, u0, u1, u2, rt,
+                          part);
+}
+
 // list of the tiles of active systems (one block; systems in order, so the list
 // is ordered and the per-system merges see their tiles in tile order)
-__global__ __launch_bounds__(256) void k_bcompact(int nsys, const int* __restrict__ tile_first,
-                                                  const int* __restrict__ active, int* __restrict__ list,
-                                                  int* __restrict__ count)
+__device__ __forceinline__ void bcompact_body(int nsys, const int* __restrict__ tile_first,
+                                              const int* __restrict__ active, int* __restrict__ list,
+                                              int* __restrict__ count)
 {
     __shared__ int base;
     __shared__ int scan[256];
@@ -592,6 +648,12 @@ __global__ __launch_bounds__(256) void k_bcompact(int nsys, const int* __restric
         __syncthreads();
     }
     if (threadIdx.x == 0) *count = base;
+}
+__global__ __launch_bounds__(256) void k_bcompact(int nsys, const int* __restrict__ tile_first,
+                                                  const int* __restrict__ active, int* __restrict__ list,
+                                                  int* __restrict__ count)
+{
+    bcompact_body(nsys, tile_first, active, list, count);
 }
 
 // dinv_i = 1 / a_ii
@@ -624,6 +686,215 @@ __global__ __launch_bounds__(256) void k_bscale(int64_t nrows, const double* __r
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
+// ---------------------------------------------------------------- one persistent launch
+// All cycles in one cooperative launch (hipLaunchCooperativeKernel: every block is
+// resident, so blocks can wait for each other): the phases of record_cycle, each
+// followed by a grid barrier, until no system is active or the cycle cap.  Same
+// phase bodies as the standalone kernels, so the iterates are the same bit for bit.
+//
+// Grid barrier.  The blocks of one XCD share its L2, and a release at agent scope
+// writes that whole L2 back, whichever block issues it; issuing one per block (a
+// thousand or more per barrier) costs far more than the phases between barriers.  So
+// the blocks are grouped by the XCD they run on (the XCC_ID hardware register, read
+// once; the group sizes are counted at the start of the launch): a block waits for
+// its own stores to reach L2 and arrives on its XCD's counter with a relaxed
+// increment; the last block of an XCD resets that counter and arrives on the top
+// counter with an acq_rel increment (one L2 write-back per XCD); the last of those
+// advances the generation word.  Waiting polls with relaxed loads and takes one
+// acquire fence once the generation has moved (an acquire in the poll loop would
+// invalidate the caches on every poll and slow the blocks still at work).  The
+// wait is bounded: after 2^24 polls (a second or more) without progress it sets the
+// error word, and a set error word releases every wait and ends the cycle loop, so
+// the launch ends even if a barrier were missed.
+constexpr int PB_XCC = 16;   // XCC_ID is a 4-bit field
+struct pbar {
+    unsigned group[PB_XCC * 16];   // arrivals per XCD, 64 bytes apart
+    unsigned reg[PB_XCC * 16];     // blocks per XCD, counted at the start
+    unsigned flat;                 // arrivals of the start-up barrier
+    unsigned pad0[15];
+    unsigned top;
+    unsigned pad1[15];
+    unsigned gen;
+    unsigned pad2[15];
+    int err;
+    int cycles;
+    int pad3[14];
+};
+constexpr unsigned PB_SPIN_LIMIT = 1u << 24;
+
+struct pgroup {
+    unsigned xcc, members, ngroups;
+};
+
+// polls until *w != v (relaxed) or the error word is set; then one acquire fence
+__device__ __forceinline__ void pb_wait_ne(pbar* P, const unsigned* w, unsigned v)
+{
+    unsigned spins = 0;
+    while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == v) {
+        if (__hip_atomic_load(&P->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins == PB_SPIN_LIMIT) {
+            __hip_atomic_store(&P->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Start of the launch: count the blocks of each XCD.  Every block registers with an
+// acq_rel increment and waits at a one-off barrier on all blocks (per-block
+// releases, once per launch).
+__device__ __forceinline__ pgroup pb_register(pbar* P)
+{
+    __shared__ pgroup g;
+    if (threadIdx.x == 0) {
+        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (PB_XCC - 1);   // HW_REG_XCC_ID
+        __hip_atomic_fetch_add(&P->reg[xcc * 16], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(&P->flat, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u)
+            __hip_atomic_store(&P->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        pb_wait_ne(P, &P->gen, 0u);
+        unsigned ng = 0;
+        for (int x = 0; x < PB_XCC; ++x)
+            ng += __hip_atomic_load(&P->reg[x * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ? 1u : 0u;
+        g.xcc = xcc;
+        g.members = __hip_atomic_load(&P->reg[xcc * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        g.ngroups = ng;
+    }
+    __syncthreads();
+    return g;
+}
+
+__device__ __forceinline__ void grid_sync(pbar* P, const pgroup& G)
+{
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have reached the XCD's L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned gen = __hip_atomic_load(&P->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(&P->group[G.xcc * 16], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            G.members - 1u) {
+            // every block of this XCD has arrived, its stores in this L2: one release
+            // writes them back for all
+            __hip_atomic_store(&P->group[G.xcc * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(&P->top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == G.ngroups - 1u) {
+                __hip_atomic_store(&P->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&P->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        pb_wait_ne(P, &P->gen, gen);
+    }
+    __syncthreads();
+}
+
+struct pargs {
+    const btile* tiles;
+    const int* tile_first;
+    int* list;
+    int* count;
+    int* active;
+    int* iters;
+    int* nactive;
+    int* confirm;
+    const int* rp;
+    const int* col;
+    const double* mval;
+    double* st;
+    double* part;
+    double* v[10];
+    int nsys;
+    int grid;    // blocks of the tile-walking SpMV phases (at most the launch's)
+    int ugrid;   // blocks of the update phases
+    int max_cycles;
+    int recalc;
+    double tol2;
+    pbar* bar;
+};
+
+// Occupancy: left alone the compiler keeps every pointer of every phase live and
+// gives the kernel 136 VGPRs (3 blocks per CU); capped at 6 waves per SIMD (80
+// VGPRs, a few cold spills) the 512-system benchmark solve takes 21.95 s against
+// 26.97 s at 3 and 22.94 s at 8.
+__global__ __launch_bounds__(BT_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_bpersistent(pargs A)
+{
+    const unsigned P = gridDim.x, b = blockIdx.x;
+    const pgroup PG = pb_register(A.bar);
+    const unsigned Gs = (unsigned)A.grid < P ? (unsigned)A.grid : P;
+    const unsigned Gu = (unsigned)A.ugrid < P ? (unsigned)A.ugrid : P;
+    double* const* v = A.v;
+    const double* none = nullptr;
+    auto sys = [&](auto nq, auto phase, auto q0, int first) {
+        bsys_body<decltype(nq)::value, decltype(phase)::value, decltype(q0)::value>(
+            b, P, A.nsys, A.tile_first, A.part, A.st, A.active, A.iters, A.nactive, A.confirm, A.tol2, first);
+        grid_sync(A.bar, PG);
+    };
+    auto upd = [&](auto kind) {
+        if (b < Gu)
+            bupd_body<decltype(kind)::value>(b, Gu, A.tiles, A.list, A.count, A.active, A.st, v[VX], v[VR0], v[VR1],
+                                             v[VR2], v[VU0], v[VU1], v[VU2], none, (double*)nullptr);
+        grid_sync(A.bar, PG);
+    };
+    auto spmv_rt = [&](const double* x, double* y) {   // y = A D x and (y, r~)
+        if (b < Gs)
+            bspmv_body<0, 1>(b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval, x, none, y,
+                             (const double*)v[VRT], none, A.part, (const int*)nullptr);
+        grid_sync(A.bar, PG);
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    using I6 = std::integral_constant<int, 6>;
+    int c = 1;
+    for (; c <= A.max_cycles; ++c) {
+        const bool recalc = A.recalc > 0 && c % A.recalc == 0;
+        if (b == 0 && threadIdx.x == 0) __hip_atomic_store(A.nactive, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sys(I1{}, I0{}, I1{}, 1);        // rho1 = (r0, r~), beta
+        upd(I0{});                       // u0 = r0 - beta u0
+        spmv_rt(v[VU0], v[VU1]);         // u1 = A D u0, (u1, r~)
+        sys(I1{}, I1{}, I0{}, 0);        // alpha
+        upd(I1{});                       // r0 -= alpha u1, x += alpha u0
+        spmv_rt(v[VR0], v[VR1]);         // r1 = A D r0, (r1, r~)
+        sys(I1{}, I0{}, I0{}, 0);
+        upd(I2{});                       // u0, u1
+        spmv_rt(v[VU1], v[VU2]);         // u2 = A D u1, (u2, r~)
+        sys(I1{}, I1{}, I0{}, 0);
+        upd(I3{});                       // r0, r1, x
+        if (b < Gs)                      // r2 = A D r1 and the Gram entries of r0, r1, r2
+            bspmv_body<0, 5>(b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval, (const double*)v[VR1],
+                             none, v[VR2], none, (const double*)v[VR0], A.part, (const int*)nullptr);
+        grid_sync(A.bar, PG);
+        sys(I5{}, I2{}, I0{}, 0);
+        auto true_resid = [&](const int* only) {
+            if (b < Gs)
+                bspmv_body<1, 2>(b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval,
+                                 (const double*)v[VX], (const double*)v[VB], v[VR0], (const double*)v[VRT], none,
+                                 A.part, only);
+            grid_sync(A.bar, PG);
+        };
+        if (recalc) {
+            upd(I4{});
+            true_resid(nullptr);
+            sys(I1{}, I3{}, I0{}, 0);
+        } else {
+            if (b < Gu)
+                bupd_body<4, true>(b, Gu, A.tiles, A.list, A.count, A.active, A.st, v[VX], v[VR0], v[VR1], v[VR2],
+                                   v[VU0], v[VU1], v[VU2], (const double*)v[VRT], A.part);
+            grid_sync(A.bar, PG);
+            sys(I1{}, I5{}, I0{}, 0);
+            true_resid(A.confirm);
+            sys(I1{}, I6{}, I0{}, 0);
+        }
+        if (b == 0) bcompact_body(A.nsys, A.tile_first, A.active, A.list, A.count);
+        grid_sync(A.bar, PG);
+        const int na = __hip_atomic_load(A.nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int err = __hip_atomic_load(&A.bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        grid_sync(A.bar, PG);   // every block has read the count before the next cycle clears it
+        if (na == 0 || err) break;
+    }
+    if (b == 0 && threadIdx.x == 0) A.bar->cycles = c <= A.max_cycles ? c : A.max_cycles;
+}
+
 // ---------------------------------------------------------------- host side
 
 static int free_batch(batch_t* B)
@@ -635,7 +906,7 @@ static int free_batch(batch_t* B)
     if (B->graph2) (void)hipGraphDestroy(B->graph2);
     void* ptrs[] = {B->d_tiles, B->d_tile_first, B->d_row_off, B->rp, B->col, B->val, B->st, B->active,
                     B->iters, B->nactive, B->part, B->dots, B->tile_list, B->tile_count, B->dinv, B->vald,
-                    B->confirm};
+                    B->confirm, B->bar};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (double* p : B->v)
@@ -785,6 +1056,7 @@ int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, dou
     M((void**)&B->tile_count, sizeof(int));
     M((void**)&B->dinv, sizeof(double) * B->nrows);
     M((void**)&B->confirm, sizeof(int) * nsys);
+    M((void**)&B->bar, sizeof(pbar));
     if (e == hipSuccess) e = hipHostMalloc((void**)&B->hcount, sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_batch(B);
@@ -856,8 +1128,8 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
                               int* out_cycles)
 {
     HPB_CLEAR_STALE();
-    HPB_REQUIRE(b && rel_tol > 0 && max_iters >= 2 && check_every >= 1, HPB_ERR_INVALID,
-                "hpb_batch_solve_bicgstab2: bad argument");
+    HPB_REQUIRE(b && rel_tol > 0 && max_iters >= 2 && check_every >= 1 && use_graph >= 0 && use_graph <= 2,
+                HPB_ERR_INVALID, "hpb_batch_solve_bicgstab2: bad argument");
     batch_t* B = (batch_t*)b;
     hpb_handle_t h = B->h;
     hipStream_t st = h->stream;
@@ -915,7 +1187,7 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     // cycle c (from 1) recomputes the true residual when recalc divides c
     const int recalc = B->recalc;
     auto is_recalc = [recalc](int c) { return recalc > 0 && c % recalc == 0; };
-    if (use_graph) {
+    if (use_graph == 1) {
         for (int k = 0; k < 2; ++k) {
             const bool rc = k == 0;
             if (rc && recalc == 0) continue;        // never used
@@ -936,11 +1208,54 @@ int hpb_batch_solve_bicgstab2(hpb_batch_t b, double rel_tol, int max_iters, int 
     auto t0 = std::chrono::steady_clock::now();
     int cycles = 0;
     const int max_cycles = max_iters / 2;
-    while (cycles < max_cycles) {
+    if (use_graph == 2) {
+        // every cycle in one cooperative launch of as many blocks as can be resident
+        pargs A{};
+        A.tiles = B->d_tiles;
+        A.tile_first = B->d_tile_first;
+        A.list = B->tile_list;
+        A.count = B->tile_count;
+        A.active = B->active;
+        A.iters = B->iters;
+        A.nactive = B->nactive;
+        A.confirm = B->confirm;
+        A.rp = B->rp;
+        A.col = B->col;
+        A.mval = B->jacobi ? B->vald : B->val;
+        A.st = B->st;
+        A.part = B->part;
+        for (int k = 0; k < 10; ++k) A.v[k] = v[k];
+        A.nsys = B->nsys;
+        A.grid = B->grid;
+        A.ugrid = B->ugrid;
+        A.max_cycles = max_cycles;
+        A.recalc = recalc;
+        A.tol2 = tol2;
+        A.bar = B->bar;
+        int per_cu = 0;
+        HPB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bpersistent, BT_THREADS, 0));
+        HPB_REQUIRE(per_cu >= 1, HPB_ERR_UNSUPPORTED, "hpb_batch_solve_bicgstab2: persistent kernel does not fit a CU");
+        const int cu = h->num_cu > 0 ? h->num_cu : 256;
+        const int P = std::min(per_cu, 8) * cu;
+        HPB_HIP(hipMemsetAsync(B->bar, 0, sizeof(pbar), st));
+        void* args[] = {&A};
+        HPB_HIP(hipLaunchCooperativeKernel((const void*)k_bpersistent, dim3((unsigned)P), dim3(BT_THREADS), args, 0,
+                                           st));
+        pbar hb;
+        HPB_HIP(hipMemcpyAsync(&hb, B->bar, sizeof(pbar), hipMemcpyDeviceToHost, st));
+        HPB_HIP(hipStreamSynchronize(st));
+        HPB_REQUIRE(hb.err == 0, HPB_ERR_UNSUPPORTED,
+                    "hpb_batch_solve_bicgstab2: the persistent launch's grid barrier made no progress for seconds");
+        cycles = hb.cycles;
+        if (B->verbose)
+            std::fprintf(stderr, "hpb_batch: one persistent launch of %d blocks, %d cycles, %.3f s\n", P, cycles,
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    while (use_graph != 2 && cycles < max_cycles) {
         const int k = std::min(check_every, max_cycles - cycles);
         for (int i = 0; i < k; ++i) {
             const bool rc = is_recalc(cycles + i + 1);
-            if (use_graph) HPB_HIP(hipGraphLaunch(rc ? B->exec : B->exec2, st));
+            if (use_graph == 1) HPB_HIP(hipGraphLaunch(rc ? B->exec : B->exec2, st));
             else record_cycle(B, tol2, rc);
         }
         HPB_LAUNCH_CHECK();

@@ -238,9 +238,11 @@ __device__ __forceinline__ void csr_row_acc(int beg, int end, int lane, const in
 constexpr int XCD_COUNT = 8;   // MI355X: 8 XCDs of 32 CUs
 struct xcd_rows {
     long first, end, stride;   // rows first + k * stride < end of this block (per row slot 0)
-    __device__ static xcd_rows of(long rows, int rows_per_block)
+    __device__ static xcd_rows of(long rows, int rows_per_block) { return of(rows, rows_per_block, gridDim.x, blockIdx.x); }
+    // the same for block b of a grid of G blocks (a persistent kernel that runs a phase
+    // on part of its grid)
+    __device__ static xcd_rows of(long rows, int rows_per_block, long G, long b)
     {
-        const long G = gridDim.x, b = blockIdx.x;
         xcd_rows R;
         if (G % XCD_COUNT != 0 || G < XCD_COUNT) {
             R.first = b * rows_per_block;

@@ -231,6 +231,9 @@ __global__ __launch_bounds__(XBLOCK) void k_exact(size_t n, const void* __restri
     __syncthreads();
     for (int w = threadIdx.x; w < XL; w += XBLOCK) st_agent_ll(&rec[w], acc[w]);
     if (threadIdx.x == 0) st_agent_ll(&rec[XL], (long long)sflag);
+    // every wave's stores complete before the barrier (the barrier alone does not wait
+    // for them), so the ticket's release below covers the whole record
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

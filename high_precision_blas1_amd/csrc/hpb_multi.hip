@@ -34,7 +34,12 @@ struct coef_pack {
     T a[HPB_MAX_MULTIDOT];
 };
 
-template <typename T, int KC, bool COMP, int BLOCK, bool VEC, bool NT = false>
+// NT: nontemporal loads of the basis columns, each read once.  w is always loaded
+// through the caches: the blocks of the other column chunks read the same w tiles
+// and find them in L2 (at n = 2^24, k = 64, streaming w as well cost 10 % of the
+// rate).  HOIST (one column chunk): the kc column loads of a tile are all issued
+// before the first Dot2 step; the order of the arithmetic is unchanged.
+template <typename T, int KC, bool COMP, int BLOCK, bool VEC, bool NT = false, bool HOIST = false>
 __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T> V, const T* __restrict__ w,
                                                     T* __restrict__ part, const int* __restrict__ skip)
 {
@@ -53,14 +58,22 @@ __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T>
     }
     const Vt* wv = reinterpret_cast<const Vt*>(w);
     for (size_t vi = (size_t)blockIdx.x * BLOCK + threadIdx.x; vi < nv; vi += (size_t)G * BLOCK) {
-        Vt a;
-        if constexpr (NT && VEC) a = dev::load_nt<Vt>(wv + vi);
-        else a = wv[vi];
+        const Vt a = wv[vi];
+        Vt bh[HOIST ? KC : 1];
+        if constexpr (HOIST) {
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) {
+                    if constexpr (NT && VEC) bh[j] = dev::load_nt<Vt>(reinterpret_cast<const Vt*>(V.p[j0 + j]) + vi);
+                    else bh[j] = reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
+                }
+        }
 #pragma unroll
         for (int j = 0; j < KC; ++j) {
             if (j < kc) {
                 Vt b;
-                if constexpr (NT && VEC) b = dev::load_nt<Vt>(reinterpret_cast<const Vt*>(V.p[j0 + j]) + vi);
+                if constexpr (HOIST) b = bh[j];
+                else if constexpr (NT && VEC) b = dev::load_nt<Vt>(reinterpret_cast<const Vt*>(V.p[j0 + j]) + vi);
                 else b = reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
 #pragma unroll
                 for (int q = 0; q < W; ++q) {
@@ -223,8 +236,20 @@ static int multidot_launch(hipStream_t st, int ncu, const hpb_tuning_t& tu, int 
     if (g > cap) g = cap;
     if (g < 1) g = 1;
     dim3 grid((unsigned)g, (unsigned)nch);
-#define HPB_MDOT(KC_, C_, V_, NT_)                                                                          \
-    hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_>), grid, dim3(256), 0, st, n, k, pk, w, part, skip)
+#define HPB_MDOT(KC_, C_, V_, NT_)                                                                               \
+    do {                                                                                                        \
+        if constexpr (V_) {                                                                                     \
+            if (nch == 1)                                                                                       \
+                hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_, true>), grid, dim3(256), 0, st, n, k, pk, \
+                                   w, part, skip);                                                              \
+            else                                                                                                \
+                hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_>), grid, dim3(256), 0, st, n, k, pk, w,    \
+                                   part, skip);                                                                 \
+        } else {                                                                                                \
+            hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_>), grid, dim3(256), 0, st, n, k, pk, w, part,  \
+                               skip);                                                                           \
+        }                                                                                                       \
+    } while (0)
 #define HPB_MDOT_KC(C_, V_, NT_)                  \
     do {                                          \
         if (KCr == 4) HPB_MDOT(4, C_, V_, NT_);   \

@@ -76,7 +76,7 @@ def test_builder_matches_single_system_builder(handle):
     H.call("hpb_batch_destroy", b)
 
 
-@pytest.mark.parametrize("use_graph,recalc", [(1, 1), (0, 1), (1, 0), (1, 3)])
+@pytest.mark.parametrize("use_graph,recalc", [(1, 1), (0, 1), (1, 0), (1, 3), (2, 1), (2, 0), (2, 3)])
 def test_batch_solve_small(handle, use_graph, recalc):
     """recalc: true residual every cycle (1), only to confirm convergence (0), every
     third cycle (3); convergence is always declared on the true residual."""
@@ -121,16 +121,43 @@ def test_batch_solve_small(handle, use_graph, recalc):
     H.call("hpb_batch_destroy", b)
 
 
-@pytest.mark.parametrize("recalc", [1, 3])
-def test_graph_and_direct_identical(handle, recalc):
+@pytest.mark.parametrize("recalc", [1, 3, 0])
+def test_graph_direct_persistent_identical(handle, recalc):
+    """Graph replay (1), direct launches (0) and the one persistent launch (2) run
+    the same phase bodies in the same order: identical iteration counts and
+    solutions, bit for bit."""
     sides = [30, 17, 45]
+    say("batch modes identical", sides, recalc)
     outs = []
-    for g in (1, 0):
+    for g in (1, 0, 2):
         b = make_batch(handle, sides)
         H.call("hpb_batch_set_resid_recalc_freq", b, recalc)
         it = np.zeros(3, dtype=np.int32)
         H.call("hpb_batch_solve_bicgstab2", b, ctypes.c_double(1e-10), 2000, g, 3, it.ctypes.data_as(ctypes.c_void_p),
                None, None, None, None)
+        x = np.empty(sum(m * m for m in sides))
+        H.call("hpb_batch_solution", b, x.ctypes.data_as(ctypes.c_void_p), None)
+        outs.append((it.copy(), x))
+        H.call("hpb_batch_destroy", b)
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0])
+        assert outs[0][1].tobytes() == o[1].tobytes()
+
+
+def test_persistent_cycle_cap(handle):
+    """The persistent launch stops at the cycle cap like the host loop does: with
+    max_iters = 10 (5 cycles) both run exactly 5 cycles and leave the same iterate."""
+    sides = [40, 25, 33]
+    say("batch persistent cycle cap", sides)
+    outs = []
+    for g in (0, 2):
+        b = make_batch(handle, sides)
+        it = np.zeros(3, dtype=np.int32)
+        cv = np.zeros(3, dtype=np.int32)
+        cyc = ctypes.c_int()
+        H.call("hpb_batch_solve_bicgstab2", b, ctypes.c_double(1e-12), 10, g, 8, it.ctypes.data_as(ctypes.c_void_p),
+               None, cv.ctypes.data_as(ctypes.c_void_p), None, ctypes.byref(cyc))
+        assert cyc.value == 5 and not cv.any(), (g, cyc.value, cv)
         x = np.empty(sum(m * m for m in sides))
         H.call("hpb_batch_solution", b, x.ctypes.data_as(ctypes.c_void_p), None)
         outs.append((it.copy(), x))
@@ -161,6 +188,8 @@ def test_jacobi_preconditioning(handle):
     it0, rr0, cv0 = _solve(handle, b, ns)
     H.call("hpb_batch_set_jacobi", b, 1)
     it1, rr1, cv1 = _solve(handle, b, ns)
+    it1p, rr1p, _ = _solve(handle, b, ns, use_graph=2)   # persistent launch, same iterates
+    assert np.array_equal(it1p, it1) and np.array_equal(rr1p, rr1)
     assert cv0.all() and cv1.all()
     assert rr1.max() <= 1e-10
     assert it1.sum() < it0.sum(), (it0, it1)
