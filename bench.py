@@ -503,9 +503,9 @@ def cfg_mixed(H, h, log, problems=("convdiff128", "aniso256")):
     return res
 
 
-def cfg_multidot(H, h, log):
+def cfg_multidot(H, h, log, ks=None):
     """Fused compensated multi-dot: k = 1..64 at n = 2^24, and lengths 2^10..2^24 for
-    k in {1, 8, 32, 64} (w read once per chunk of 8 columns)."""
+    k in {1, 8, 32, 64}.  ks (a profiling aid): only these k at n = 2^24, no sweep."""
     import numpy as np
     log("multidot")
     nmax = 1 << 24
@@ -517,14 +517,14 @@ def cfg_multidot(H, h, log):
     d = H.DeviceArray(4 * kmax, np.float64, h)
     L = H.lib()
     rows = []
-    for k in (1, 2, 4, 8, 16, 32, 50, 64):
+    for k in ks or (1, 2, 4, 8, 16, 32, 50, 64):
         ptrs = (ctypes.c_void_p * k)(*[a.addr for a in V[:k]])
         ms = timed(h, L.hpb_multidot_f64_async, (h.ptr, k, ptrs, nmax, w.ptr, 1, d.ptr), 10, 2)
         nbytes = 8.0 * nmax * (k + 1)
         rows.append({"k": k, "ms": round(ms, 4), "gbs_algorithmic": round(nbytes / ms / 1e6, 1),
                      "separate_dot2_bytes_ratio": round(2.0 * k / (k + 1), 3)})
     sweep = []
-    for e in (10, 13, 16, 19, 22, 24):
+    for e in () if ks else (10, 13, 16, 19, 22, 24):
         n = 1 << e
         for k in (1, 8, 32, 64):
             ptrs = (ctypes.c_void_p * k)(*[a.addr for a in V[:k]])
@@ -711,7 +711,8 @@ def run_suite(H, h, x, y, n, args, log, no_cpu=False, only=None):
     if probs:
         res["mixed"] = cfg_mixed(H, h, log, probs)
     if "multidot" in want:
-        res["multidot"] = cfg_multidot(H, h, log)
+        ks = [int(v) for v in args.multidot_k.split(",")] if args.multidot_k else None
+        res["multidot"] = cfg_multidot(H, h, log, ks)
     if "batch" in want:
         mode = "direct" if args.batch_direct else args.batch_launch
         res["batch"] = cfg_batch(H, h, log, use_graph=BATCH_LAUNCH[mode])
@@ -734,6 +735,8 @@ def main():
                     help="kernel timed for the headline (copy: the plain copy used to calibrate HBM counters)")
     ap.add_argument("--batch-direct", action="store_true",
                     help="batch suite with directly launched cycles instead of graph replays (profiling aid)")
+    ap.add_argument("--multidot-k", default=None,
+                    help="multidot suite: only these comma-separated k at n = 2^24 (profiling aid)")
     ap.add_argument("--batch-launch", default="graph", choices=tuple(BATCH_LAUNCH),
                     help="how the batch suite runs its cycles: graph replays, direct launches, or one persistent "
                          "cooperative launch")
