@@ -92,7 +92,7 @@ def profiled_traffic(n, cfg, nontemporal):
         if line.startswith("traffic_bytes_per_launch:"):
             fields["traffic"] = int(line.split(":")[1])
         elif line.startswith("launch:"):
-            fields.update(kv.split("=") for kv in line.split(":", 1)[1].split())
+            fields.update(kv.split("=", 1) for kv in line.split(":", 1)[1].split() if "=" in kv)
     want = {"n": n, "block": cfg[0], "unroll": cfg[1], "grid": cfg[2], "vector_width": cfg[3],
             "nontemporal": int(bool(nontemporal))}
     if "traffic" not in fields or any(int(fields.get(k, -1)) != v for k, v in want.items()):
