@@ -10,15 +10,20 @@
 // Design:
 //   * a fixed-point long accumulator of XL 32-bit digits held in int64 words;
 //     bit 0 is 2^-1074, so every finite double is an integer multiple of it;
-//   * each thread streams its elements into a small floating-point expansion of
-//     K doubles (a TwoSum cascade, error-free); what falls out of the last
-//     component is added to the block's accumulator in LDS with 64-bit integer
-//     atomics, and the expansion itself is added at the end;
+//   * each thread streams its elements into two floating-point expansions of K
+//     doubles (TwoSum cascades, error-free; two, so that the dependency chains of
+//     consecutive terms interleave); what falls out of the last component is added
+//     to the block's accumulator in LDS with 64-bit integer atomics (for the
+//     generator's pairs up to kappa 1e16 this never happens at K = 6), and the
+//     expansions themselves are added at the end;
 //   * products are split exactly with TwoProd (p + e = x*y); FP32 products are
 //     exact in FP64 and added as one double;
-//   * each block carries its accumulator to 32-bit digits and stores it; the block
-//     that takes the last ticket adds the digits of all blocks (integer sums, so
-//     the order does not matter), carries, and rounds once to nearest even.
+//   * each block carries its accumulator to 32-bit digits and stores it; a second
+//     launch adds the digits of all blocks (one block per digit; integer sums, so
+//     the order does not matter), and its last block carries and rounds once to
+//     nearest even.  (Formerly the last block of the first launch added all the records
+//     itself, one digit per thread over every block in turn: at 2^27 elements
+//     that serial walk was half the time.)
 // Domain (reported, never silently wrong): finite inputs; for FP64 dots every
 // nonzero product has |x*y| >= 2^-969 (below that TwoProd's error term can
 // underflow) and every |x*y| (|x| for sums) < 2^960.  Out-of-domain inputs make the
@@ -32,7 +37,6 @@ namespace hpb {
 namespace {
 
 constexpr int XL = 72;    // 72 * 32 = 2304 bits >= 2098 + carry headroom
-constexpr int XP = 80;    // words per block record: XL digits, flag, padding
 constexpr int XF_NONFINITE = 1;
 constexpr int XF_RANGE = 2;
 constexpr int XBLOCK = 256;
@@ -134,11 +138,15 @@ __device__ double acc_round(const long long* d, unsigned* mag)
     return neg ? -r : r;
 }
 
+// Adds v to the expansion a[0..K) with a TwoSum cascade (error-free); what falls
+// out of the last term goes to the block's accumulator.  A zero error ends the
+// cascade: the remaining terms would not change.
 template <int K>
 __device__ __forceinline__ void fpe_add(double (&a)[K], double v, long long* acc)
 {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
+        if (v == 0.0) return;
         double t;
         a[j] = dev::two_sum<double>(a[j], v, t);
         v = t;
@@ -149,144 +157,200 @@ __device__ __forceinline__ void fpe_add(double (&a)[K], double v, long long* acc
 constexpr double TWO_M969 = 0x1p-969;
 constexpr double TWO_P960 = 0x1p+960;
 
+// One element into the thread's two expansions a[0], a[1]: for FP64 dots the
+// product goes to a[0] and its TwoProd error to a[1] (two independent dependency
+// chains); FP32 products and summands alternate by element (odd).  The range flags
+// are formed without branches and the element is added regardless: a non-finite or
+// out-of-range term only spoils a value that its flag then withholds (non-finite
+// digits are skipped by acc_add).
 template <int K, int MODE>
-__device__ __forceinline__ void x_element(double (&a)[K], long long* acc, int& flag, const void* xv, const void* yv,
-                                          size_t i)
+__device__ __forceinline__ void x_element(double (&a)[2][K], long long* acc, int& flag, double x, double y, int odd)
 {
     if constexpr (MODE == X_DOT64) {
-        const double x = static_cast<const double*>(xv)[i];
-        const double y = static_cast<const double*>(yv)[i];
         const double p = x * y;
         const double e = __builtin_fma(x, y, -p);
         const double ap = __builtin_fabs(p);
-        if (!(ap < TWO_P960)) {
-            flag |= (__builtin_isfinite(x) && __builtin_isfinite(y)) ? XF_RANGE : XF_NONFINITE;
-            return;
-        }
-        if (ap < TWO_M969 && x != 0.0 && y != 0.0) flag |= XF_RANGE;
-        fpe_add<K>(a, p, acc);
-        fpe_add<K>(a, e, acc);
+        const int big = !(ap < TWO_P960) ? ((__builtin_isfinite(x) && __builtin_isfinite(y)) ? XF_RANGE : XF_NONFINITE)
+                                         : 0;
+        const int tiny = (ap < TWO_M969 && x != 0.0 && y != 0.0) ? XF_RANGE : 0;
+        flag |= big | tiny;
+        fpe_add<K>(a[0], p, acc);
+        fpe_add<K>(a[1], e, acc);
     } else if constexpr (MODE == X_DOT32) {
-        const double x = (double)static_cast<const float*>(xv)[i];
-        const double y = (double)static_cast<const float*>(yv)[i];
         const double p = x * y;   // 24 x 24 bits: exact
-        if (!__builtin_isfinite(p)) {
-            flag |= XF_NONFINITE;
-            return;
-        }
-        fpe_add<K>(a, p, acc);
+        flag |= __builtin_isfinite(p) ? 0 : XF_NONFINITE;
+        fpe_add<K>(a[odd], p, acc);
     } else {
-        const double x = static_cast<const double*>(xv)[i];
-        if (!(__builtin_fabs(x) < TWO_P960)) {
-            flag |= __builtin_isfinite(x) ? XF_RANGE : XF_NONFINITE;
-            return;
-        }
-        fpe_add<K>(a, x, acc);
+        flag |= !(__builtin_fabs(x) < TWO_P960) ? (__builtin_isfinite(x) ? XF_RANGE : XF_NONFINITE) : 0;
+        fpe_add<K>(a[odd], x, acc);
     }
 }
 
-__device__ __forceinline__ void st_agent_ll(long long* p, long long v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ long long ld_agent_ll(const long long* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+template <int MODE> struct x_elem_t { using type = double; };
+template <> struct x_elem_t<X_DOT32> { using type = float; };
 
-// out[0] = hi, out[1] = lo, out[2] = flags (as a double).
-template <int K, int MODE>
+// Per-block digit records.  Each thread streams its elements (16-byte vector loads
+// when VEC, two vectors in flight, their elements formed in an unrolled loop) into
+// two K-term expansions; their terms and the rare overflow of the cascades go to the
+// block's accumulator in LDS (64-bit integer atomics: the order of the additions
+// does not matter).  The block carries its accumulator to 32-bit digits and stores
+// them digit-major: part[w G + block] for w < XL, the input flags in
+// part[XL G + block].
+template <int K, int MODE, bool VEC>
 __global__ __launch_bounds__(XBLOCK) void k_exact(size_t n, const void* __restrict__ xv, const void* __restrict__ yv,
-                                                  long long* __restrict__ part, unsigned* __restrict__ ticket,
-                                                  double* __restrict__ out)
+                                                  long long* __restrict__ part)
 {
+    using T = typename x_elem_t<MODE>::type;
     __shared__ long long acc[XL];
     __shared__ int sflag;
-    __shared__ int am_last;
-    __shared__ unsigned mag[XL];
     for (int i = threadIdx.x; i < XL; i += XBLOCK) acc[i] = 0;
     if (threadIdx.x == 0) sflag = 0;
     __syncthreads();
 
-    double a[K];
+    double a[2][K];
 #pragma unroll
-    for (int j = 0; j < K; ++j) a[j] = 0.0;
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int j = 0; j < K; ++j) a[q][j] = 0.0;
     int flag = 0;
+    const T* x = static_cast<const T*>(xv);
+    const T* y = static_cast<const T*>(yv);   // null for sums
+    auto load_y = [&](size_t i) -> double {
+        if constexpr (MODE == X_SUM64) return 0.0;
+        else return (double)y[i];
+    };
     const size_t stride = (size_t)gridDim.x * XBLOCK;
-    size_t i = (size_t)blockIdx.x * XBLOCK + threadIdx.x;
-    // two elements in flight per thread
-    for (; i + stride < n; i += 2 * stride) {
-        x_element<K, MODE>(a, acc, flag, xv, yv, i);
-        x_element<K, MODE>(a, acc, flag, xv, yv, i + stride);
-    }
-    if (i < n) x_element<K, MODE>(a, acc, flag, xv, yv, i);
+    if constexpr (VEC) {
+        using V = typename dev::vec16<T>::type;
+        constexpr int W = dev::vec16<T>::W;
+        const V* xq = reinterpret_cast<const V*>(x);
+        const V* yq = reinterpret_cast<const V*>(y);
+        const size_t nv = n / W;
+        size_t i = (size_t)blockIdx.x * XBLOCK + threadIdx.x;
+        for (; i + stride < nv; i += 2 * stride) {
+            const V x0 = xq[i], x1 = xq[i + stride];
+            double xs[2 * W], ys[2 * W];
 #pragma unroll
-    for (int j = 0; j < K; ++j) acc_add<true>(acc, a[j]);
+            for (int q = 0; q < W; ++q) {
+                xs[q] = (double)dev::vget<T>(x0, q);
+                xs[W + q] = (double)dev::vget<T>(x1, q);
+                ys[q] = ys[W + q] = 0.0;
+            }
+            if constexpr (MODE != X_SUM64) {
+                const V y0 = yq[i], y1 = yq[i + stride];
+#pragma unroll
+                for (int q = 0; q < W; ++q) {
+                    ys[q] = (double)dev::vget<T>(y0, q);
+                    ys[W + q] = (double)dev::vget<T>(y1, q);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2 * W; ++u) x_element<K, MODE>(a, acc, flag, xs[u], ys[u], u & 1);
+        }
+        if (i < nv) {
+            const V x0 = xq[i];
+            V y0{};
+            if constexpr (MODE != X_SUM64) y0 = yq[i];
+#pragma unroll
+            for (int q = 0; q < W; ++q)
+                x_element<K, MODE>(a, acc, flag, (double)dev::vget<T>(x0, q), (double)dev::vget<T>(y0, q), q & 1);
+        }
+        const size_t r0 = nv * W;   // the last n mod W elements
+        if (blockIdx.x == 0 && threadIdx.x < n - r0)
+            x_element<K, MODE>(a, acc, flag, (double)x[r0 + threadIdx.x], load_y(r0 + threadIdx.x), 0);
+    } else {
+        size_t i = (size_t)blockIdx.x * XBLOCK + threadIdx.x;
+        for (; i + stride < n; i += 2 * stride) {
+            x_element<K, MODE>(a, acc, flag, (double)x[i], load_y(i), 0);
+            x_element<K, MODE>(a, acc, flag, (double)x[i + stride], load_y(i + stride), 1);
+        }
+        if (i < n) x_element<K, MODE>(a, acc, flag, (double)x[i], load_y(i), 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc_add<true>(acc, a[q][j]);
     if (flag) atomicOr(&sflag, flag);
     __syncthreads();
-
-    const unsigned G = gridDim.x;
-    long long* rec = part + (size_t)blockIdx.x * XP;
     if (threadIdx.x == 0) acc_carry(acc);
     __syncthreads();
-    for (int w = threadIdx.x; w < XL; w += XBLOCK) st_agent_ll(&rec[w], acc[w]);
-    if (threadIdx.x == 0) st_agent_ll(&rec[XL], (long long)sflag);
-    // every wave's stores complete before the barrier (the barrier alone does not wait
-    // for them), so the ticket's release below covers the whole record
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == G - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-
-    // Digit sums over all blocks: each carried digit is < 2^32 (top word small
-    // and signed), so G <= 2^20 blocks cannot overflow int64.
-    for (int w = threadIdx.x; w <= XL; w += XBLOCK) {
-        long long s = 0;
-        if (w < XL) {
-            for (unsigned b = 0; b < G; ++b) s += ld_agent_ll(&part[(size_t)b * XP + w]);
-            acc[w] = s;
-        } else {
-            for (unsigned b = 0; b < G; ++b) s |= ld_agent_ll(&part[(size_t)b * XP + XL]);
-            sflag = (int)s;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double hi = __builtin_nan(""), lo = __builtin_nan("");
-        if (sflag == 0) {
-            acc_carry(acc);
-            hi = acc_round(acc, mag);
-            if (__builtin_isfinite(hi)) {
-                acc_add<false>(acc, -hi);
-                acc_carry(acc);
-                lo = acc_round(acc, mag);
-            } else {
-                lo = 0.0;
-            }
-        }
-        out[0] = hi;
-        out[1] = lo;
-        out[2] = (double)sflag;
-    }
+    const unsigned G = gridDim.x;
+    for (int w = threadIdx.x; w < XL; w += XBLOCK) part[(size_t)w * G + blockIdx.x] = acc[w];
+    if (threadIdx.x == 0) part[(size_t)XL * G + blockIdx.x] = sflag;
 }
 
-constexpr int XK = 4;   // expansion length per thread
+// Digit sums over the G block records, one block per word (word XL: the flags,
+// OR-ed).  Each carried digit is < 2^32 (the top word small and signed), so G <= 2^20
+// blocks cannot overflow int64; integer sums, so the order does not matter.  The
+// block that takes the last ticket (each block's sum published by a release
+// increment) carries the sums, rounds to nearest even, subtracts, rounds again:
+// out[0] = hi, out[1] = lo, out[2] = flags (as a double).
+__global__ __launch_bounds__(256) void k_exact_digits(unsigned G, const long long* __restrict__ part,
+                                                      long long* __restrict__ sums, unsigned* __restrict__ ticket,
+                                                      double* __restrict__ out)
+{
+    __shared__ long long red[256];
+    __shared__ int am_last;
+    __shared__ long long acc[XL];
+    __shared__ unsigned mag[XL];
+    const unsigned w = blockIdx.x;
+    const long long* p = part + (size_t)w * G;
+    long long s = 0;
+    for (unsigned k = threadIdx.x; k < G; k += 256) s = w < (unsigned)XL ? s + p[k] : (s | p[k]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) red[threadIdx.x] = w < (unsigned)XL ? red[threadIdx.x] + red[threadIdx.x + m]
+                                                                     : (red[threadIdx.x] | red[threadIdx.x + m]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    sums[w] = red[0];
+    am_last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u;
+    if (!am_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int d = 0; d < XL; ++d) acc[d] = sums[d];
+    const int sflag = (int)sums[XL];
+    double hi = __builtin_nan(""), lo = __builtin_nan("");
+    if (sflag == 0) {
+        acc_carry(acc);
+        hi = acc_round(acc, mag);
+        if (__builtin_isfinite(hi)) {
+            acc_add<false>(acc, -hi);
+            acc_carry(acc);
+            lo = acc_round(acc, mag);
+        } else {
+            lo = 0.0;
+        }
+    }
+    out[0] = hi;
+    out[1] = lo;
+    out[2] = (double)sflag;
+}
+
+constexpr int XK = 6;   // terms per expansion (two expansions per thread)
 
 template <int MODE>
 int exact_launch(hpb_handle_t h, size_t n, const void* x, const void* y, double* dev_out3)
 {
-    size_t want = (n + 2 * (size_t)XBLOCK * 8 - 1) / (2 * (size_t)XBLOCK * 8);   // >= 8 elements per thread
+    using T = typename x_elem_t<MODE>::type;
+    size_t want = (n + (size_t)XBLOCK * 16 - 1) / ((size_t)XBLOCK * 16);   // >= 16 elements per thread
     if (want < 1) want = 1;
-    int grid = (int)(want < (size_t)XGRID_MAX ? want : (size_t)XGRID_MAX);
-    if ((size_t)grid * XP * sizeof(long long) > h->part_bytes) grid = (int)(h->part_bytes / (XP * sizeof(long long)));
-    hipLaunchKernelGGL((k_exact<XK, MODE>), dim3(grid), dim3(XBLOCK), 0, h->stream, n, x, y,
-                       reinterpret_cast<long long*>(h->part), h->ticket + 2, dev_out3);
+    size_t grid = want < (size_t)XGRID_MAX ? want : (size_t)XGRID_MAX;
+    // records ((XL + 1) G words) and the digit sums (XL + 1 words) in the workspace
+    const size_t gcap = h->part_bytes / ((XL + 1) * sizeof(long long)) - 1;
+    if (grid > gcap) grid = gcap;
+    const unsigned G = (unsigned)grid;
+    long long* part = reinterpret_cast<long long*>(h->part);
+    long long* sums = part + (size_t)(XL + 1) * G;
+    const bool vec = ((uintptr_t)x & 15u) == 0 && (MODE == X_SUM64 || ((uintptr_t)y & 15u) == 0);
+    if (vec)
+        hipLaunchKernelGGL((k_exact<XK, MODE, true>), dim3(G), dim3(XBLOCK), 0, h->stream, n, x, y, part);
+    else
+        hipLaunchKernelGGL((k_exact<XK, MODE, false>), dim3(G), dim3(XBLOCK), 0, h->stream, n, x, y, part);
+    hipLaunchKernelGGL(k_exact_digits, dim3(XL + 1), dim3(256), 0, h->stream, G, part, sums, h->ticket + 2, dev_out3);
+    (void)sizeof(T);
     return HPB_OK;
 }
 

@@ -154,3 +154,28 @@ def test_exact_dot_2p27_generator(handle):
     assert out[0] == d3[0]
     dx.free()
     dy.free()
+
+
+def test_exact_dot_unaligned_and_wide_range(handle):
+    """Vectors starting 8 bytes past a 16-byte boundary (the scalar-load kernel), and
+    products spread over 2^-880..2^880 so that the per-thread expansions overflow into
+    the block accumulator many times; both against the Kulisch oracle."""
+    r = np.random.default_rng(21)
+    n = 300001
+    x = r.standard_normal(n) * np.exp2(r.integers(-440, 440, n))
+    y = r.standard_normal(n) * np.exp2(r.integers(-440, 440, n))
+    ref = O.exact_dot(x, y)
+    say("exact dot wide range", n)
+    assert _exact_dot(handle, x, y) == ref
+    dx = H.DeviceArray.from_numpy(np.concatenate([[0.0], x]), handle)
+    dy = H.DeviceArray.from_numpy(np.concatenate([[0.0], y]), handle)
+    out = np.zeros(2)
+    say("exact dot unaligned", n)
+    H.call("hpb_exact_dot_f64", handle.ptr, n, ctypes.c_void_p(dx.addr + 8), ctypes.c_void_p(dy.addr + 8),
+           out.ctypes.data)
+    assert (float(out[0]), float(out[1])) == ref
+    out = np.zeros(2)
+    H.call("hpb_exact_sum_f64", handle.ptr, n, ctypes.c_void_p(dx.addr + 8), out.ctypes.data)
+    assert float(out[0]) == math.fsum(x) == O.exact_sum(x)
+    dx.free()
+    dy.free()
