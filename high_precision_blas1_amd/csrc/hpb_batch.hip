@@ -182,8 +182,13 @@ __global__ __launch_bounds__(256) void k_batch_build(int nsys, int64_t nrows, co
 //   EPI 5: the Gram entries of the MR step with y = r2, x = r1 and z = r0:
 //          (r1, r1), (r2, r1), (r2, r2), (r0, r1), (r0, r2) in slots 0..4.
 // With only != null the tiles of systems with only[sys] == 0 are skipped.
+constexpr int BSPMV_LPR = 2;                         // lanes per row of the batch SpMV
+constexpr int BSPMV_SRT = BT_THREADS / BSPMV_LPR;    // staged sub-tile: rows (one per lane group)
+constexpr int BSPMV_SCAP = 1024;                     // staged sub-tile: entries
+using bstage = dev::csr_stage<double, BSPMV_SRT, BSPMV_SCAP>;
+
 template <int MODE, int EPI>
-__device__ __forceinline__ void bspmv_body(unsigned bid, unsigned nblk, const btile* __restrict__ tiles,
+__device__ __forceinline__ void bspmv_body(bstage& S, unsigned bid, unsigned nblk, const btile* __restrict__ tiles,
                                            const int* __restrict__ list, const int* __restrict__ count,
                                            const int* __restrict__ active, const int* __restrict__ rp,
                                            const int* __restrict__ col, const double* __restrict__ val,
@@ -199,6 +204,10 @@ __device__ __forceinline__ void bspmv_body(unsigned bid, unsigned nblk, const bt
     // barrier per tile suffices: buffer k is read after the barrier of its tile and
     // written again only two tiles later, after every thread passed the next barrier
     __shared__ double wtot[2][NQ][NW][2];
+    // lanes per row: on these 5-entry rows 2 lanes beat 4 by 8 % over a whole
+    // cycle (1 lane: 8 % slower than 4, 8 lanes: 18 % slower)
+    constexpr int LPR = BSPMV_LPR;
+    constexpr int SRT = BSPMV_SRT, SCAP = BSPMV_SCAP;
     int buf = 0;
     const int nt = *count;
     const dev::xcd_rows Q = dev::xcd_rows::of(nt, 1, nblk, bid);
@@ -206,47 +215,60 @@ __device__ __forceinline__ void bspmv_body(unsigned bid, unsigned nblk, const bt
         const int ti = list[q];
         const btile t = tiles[ti];
         if (!active[t.sys] || (only != nullptr && !only[t.sys])) continue;
-        // lanes per row: on these 5-entry rows 2 lanes beat 4 by 8 % over a whole
-        // cycle (1 lane: 8 % slower than 4, 8 lanes: 18 % slower)
-        constexpr int LPR = 2;
         const int lane = threadIdx.x % LPR;
         double ds[NQ], dc[NQ];
 #pragma unroll
         for (int k = 0; k < NQ; ++k) ds[k] = dc[k] = 0.0;
-        for (int64_t r = t.r0 + threadIdx.x / LPR; r < t.r1; r += BT_THREADS / LPR) {
-            // the epilogue's own operands, loaded ahead of the row's gathers
-            double e0 = 0.0, e1 = 0.0;
-            if (EPI != 0 && lane == 0) {
-                if (EPI == 5) {
-                    e0 = x[r];
-                    e1 = z[r];
-                } else {
-                    e0 = rt[r];
+        // sub-tiles of SRT rows, one per lane group, staged in LDS (csr_stage_tile):
+        // each thread keeps the rows t.r0 + threadIdx.x / LPR + k SRT of the direct walk
+        for (int64_t r0 = t.r0; r0 < t.r1; r0 += SRT) {
+            const int nr = (int)((t.r1 - r0) < SRT ? (t.r1 - r0) : SRT);
+            const bool staged = dev::csr_stage_tile<double, SRT, SCAP, BT_THREADS>(S, r0, nr, rp, col, val, x);
+            const int eb = staged ? S.rp[0] : 0;
+            const int rr = threadIdx.x / LPR;
+            // (the barrier at the end of the sub-tile stays outside this branch: a wave
+            // whose lanes split between two barriers would execute both)
+            if (rr < nr) {
+                const int64_t r = r0 + rr;
+                // the epilogue's own operands, loaded ahead of the row's gathers
+                double e0 = 0.0, e1 = 0.0;
+                if (EPI != 0 && lane == 0) {
+                    if (EPI == 5) {
+                        e0 = x[r];
+                        e1 = z[r];
+                    } else {
+                        e0 = rt[r];
+                    }
                 }
-            }
-            double s = (MODE == 1 && lane == 0) ? b[r] : 0.0, c = 0.0;
-            dev::csr_row_acc<double, LPR, true, MODE == 1>(rp[r], rp[r + 1], lane, col, val, x, s, c);
+                double s = (MODE == 1 && lane == 0) ? b[r] : 0.0, c = 0.0;
+                if (staged)
+                    dev::csr_row_acc_staged<double, LPR, true, MODE == 1>(S.rp[rr] - eb, S.rp[rr + 1] - eb, lane, S.v,
+                                                                         S.x, s, c);
+                else
+                    dev::csr_row_acc<double, LPR, true, MODE == 1>(S.rp[rr], S.rp[rr + 1], lane, col, val, x, s, c);
 #pragma unroll
-            for (int m = LPR / 2; m >= 1; m >>= 1) {
-                double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
-                dev::merge<double>(s, c, s2, c2);
-            }
-            if (lane == 0) {
-                const double yv = s + c;
-                y[r] = yv;
-                if (EPI == 1) {
-                    dev::dot2_step<double>(ds[0], dc[0], yv, e0);
-                } else if (EPI == 2) {
-                    dev::dot2_step<double>(ds[0], dc[0], yv, yv);
-                    dev::dot2_step<double>(ds[NQ - 1], dc[NQ - 1], yv, e0);
-                } else if (EPI == 5) {   // e0 = r1, e1 = r0, yv = r2
-                    dev::dot2_step<double>(ds[0], dc[0], e0, e0);
-                    dev::dot2_step<double>(ds[1 % NQ], dc[1 % NQ], yv, e0);
-                    dev::dot2_step<double>(ds[2 % NQ], dc[2 % NQ], yv, yv);
-                    dev::dot2_step<double>(ds[3 % NQ], dc[3 % NQ], e1, e0);
-                    dev::dot2_step<double>(ds[4 % NQ], dc[4 % NQ], e1, yv);
+                for (int m = LPR / 2; m >= 1; m >>= 1) {
+                    double s2 = __shfl_xor(s, m, 64), c2 = __shfl_xor(c, m, 64);
+                    dev::merge<double>(s, c, s2, c2);
+                }
+                if (lane == 0) {
+                    const double yv = s + c;
+                    y[r] = yv;
+                    if (EPI == 1) {
+                        dev::dot2_step<double>(ds[0], dc[0], yv, e0);
+                    } else if (EPI == 2) {
+                        dev::dot2_step<double>(ds[0], dc[0], yv, yv);
+                        dev::dot2_step<double>(ds[NQ - 1], dc[NQ - 1], yv, e0);
+                    } else if (EPI == 5) {   // e0 = r1, e1 = r0, yv = r2
+                        dev::dot2_step<double>(ds[0], dc[0], e0, e0);
+                        dev::dot2_step<double>(ds[1 % NQ], dc[1 % NQ], yv, e0);
+                        dev::dot2_step<double>(ds[2 % NQ], dc[2 % NQ], yv, yv);
+                        dev::dot2_step<double>(ds[3 % NQ], dc[3 % NQ], e1, e0);
+                        dev::dot2_step<double>(ds[4 % NQ], dc[4 % NQ], e1, yv);
+                    }
                 }
             }
+            __syncthreads();   // the next sub-tile overwrites S
         }
         if (EPI != 0) {
             const int w = threadIdx.x >> 6;
@@ -282,7 +304,9 @@ __global__ __launch_bounds__(BT_THREADS) void k_bspmv(const btile* __restrict__ 
                                                       const double* __restrict__ z, double* __restrict__ part,
                                                       const int* __restrict__ only)
 {
-    bspmv_body<MODE, EPI>(blockIdx.x, gridDim.x, tiles, list, count, active, rp, col, val, x, b, y, rt, z, part, only);
+    __shared__ bstage S;
+    bspmv_body<MODE, EPI>(S, blockIdx.x, gridDim.x, tiles, list, count, active, rp, col, val, x, b, y, rt, z, part,
+                          only);
 }
 
 struct dot_pairs {
@@ -817,6 +841,7 @@ __global__ __launch_bounds__(BT_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
 {
     const unsigned P = gridDim.x, b = blockIdx.x;
     const pgroup PG = pb_register(A.bar);
+    __shared__ bstage S;   // one staging buffer for the three SpMV phases
     const unsigned Gs = (unsigned)A.grid < P ? (unsigned)A.grid : P;
     const unsigned Gu = (unsigned)A.ugrid < P ? (unsigned)A.ugrid : P;
     double* const* v = A.v;
@@ -834,7 +859,7 @@ __global__ __launch_bounds__(BT_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
     };
     auto spmv_rt = [&](const double* x, double* y) {   // y = A D x and (y, r~)
         if (b < Gs)
-            bspmv_body<0, 1>(b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval, x, none, y,
+            bspmv_body<0, 1>(S, b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval, x, none, y,
                              (const double*)v[VRT], none, A.part, (const int*)nullptr);
         grid_sync(A.bar, PG);
     };
@@ -861,13 +886,13 @@ __global__ __launch_bounds__(BT_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
         sys(I1{}, I1{}, I0{}, 0);
         upd(I3{});                       // r0, r1, x
         if (b < Gs)                      // r2 = A D r1 and the Gram entries of r0, r1, r2
-            bspmv_body<0, 5>(b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval, (const double*)v[VR1],
+            bspmv_body<0, 5>(S, b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval, (const double*)v[VR1],
                              none, v[VR2], none, (const double*)v[VR0], A.part, (const int*)nullptr);
         grid_sync(A.bar, PG);
         sys(I5{}, I2{}, I0{}, 0);
         auto true_resid = [&](const int* only) {
             if (b < Gs)
-                bspmv_body<1, 2>(b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval,
+                bspmv_body<1, 2>(S, b, Gs, A.tiles, A.list, A.count, A.active, A.rp, A.col, A.mval,
                                  (const double*)v[VX], (const double*)v[VB], v[VR0], (const double*)v[VRT], none,
                                  A.part, only);
             grid_sync(A.bar, PG);

@@ -25,6 +25,12 @@ namespace hpb {
 
 constexpr int SPMV_BLOCK = 256;
 
+// Staged tiles (csr_stage_tile) for up to 4 lanes per row: RT rows, CAP entries
+// (mean row length up to 2 for one lane, 8 for two or four); longer rows read their
+// entries directly.
+template <int LPR> struct spmv_tile { static constexpr int RT = LPR == 1 ? 256 : 128, CAP = LPR == 1 ? 512 : 1024; };
+template <int LPR> constexpr bool spmv_staged() { return LPR <= 4; }
+
 template <typename T, int LPR, bool COMP, int MODE>
 __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __restrict__ rp,
                                                      const int* __restrict__ col, const T* __restrict__ val,
@@ -36,13 +42,7 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
     if (skip != nullptr && *skip) return;   // solver graphs: flag set, launch does nothing
     if (dalpha != nullptr) alpha = *dalpha;  // solver graphs: scale factor from device memory
     const int lane = threadIdx.x % LPR;
-    const dev::xcd_rows R = dev::xcd_rows::of(rows, SPMV_BLOCK / LPR);
-    for (long r = R.first + (long)threadIdx.x / LPR; r < R.end; r += R.stride) {
-        const int beg = rp[r];
-        const int end = rp[r + 1];
-        T s = T(0), c = T(0);
-        if (MODE == 1 && lane == 0) s = b[r];
-        dev::csr_row_acc<T, LPR, COMP, MODE == 1>(beg, end, lane, col, val, x, s, c);
+    auto finish = [&](long r, T s, T c) {
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1) {
             T s2 = __shfl_xor(s, m, 64);
@@ -58,6 +58,35 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
             if (MODE == 1) y[r] = v;
             else if (beta == T(0)) y[r] = (alpha == T(1)) ? v : alpha * v;
             else y[r] = dev::fma_rn<T>(alpha, v, beta * y[r]);
+        }
+    };
+    if constexpr (spmv_staged<LPR>()) {
+        constexpr int RT = spmv_tile<LPR>::RT, CAP = spmv_tile<LPR>::CAP;
+        __shared__ dev::csr_stage<T, RT, CAP> S;
+        const dev::xcd_rows R = dev::xcd_rows::of(rows, RT);
+        for (long r0 = R.first; r0 < R.end; r0 += R.stride) {
+            const int nr = (int)((R.end - r0) < RT ? (R.end - r0) : RT);
+            const bool staged = dev::csr_stage_tile<T, RT, CAP, SPMV_BLOCK>(S, r0, nr, rp, col, val, x);
+            const int e0 = S.rp[0];
+            for (int rr = threadIdx.x / LPR; rr < nr; rr += SPMV_BLOCK / LPR) {
+                T s = T(0), c = T(0);
+                if (MODE == 1 && lane == 0) s = b[r0 + rr];
+                if (staged)
+                    dev::csr_row_acc_staged<T, LPR, COMP, MODE == 1>(S.rp[rr] - e0, S.rp[rr + 1] - e0, lane, S.v, S.x,
+                                                                    s, c);
+                else
+                    dev::csr_row_acc<T, LPR, COMP, MODE == 1>(S.rp[rr], S.rp[rr + 1], lane, col, val, x, s, c);
+                finish(r0 + rr, s, c);
+            }
+            __syncthreads();   // the next tile overwrites S
+        }
+    } else {
+        const dev::xcd_rows R = dev::xcd_rows::of(rows, SPMV_BLOCK / LPR);
+        for (long r = R.first + (long)threadIdx.x / LPR; r < R.end; r += R.stride) {
+            T s = T(0), c = T(0);
+            if (MODE == 1 && lane == 0) s = b[r];
+            dev::csr_row_acc<T, LPR, COMP, MODE == 1>(rp[r], rp[r + 1], lane, col, val, x, s, c);
+            finish(r, s, c);
         }
     }
 }
@@ -99,7 +128,7 @@ static int spmv_entry(hpb_handle_t h, int rows, int cols, long nnz, const int* r
     if (lpr <= 0) lpr = pick_lpr(rows, nnz);
     HPB_REQUIRE(lpr == 1 || lpr == 2 || lpr == 4 || lpr == 8 || lpr == 16 || lpr == 32 || lpr == 64,
                 HPB_ERR_INVALID, "csr spmv: lanes per row must be a power of two <= 64");
-    const long rows_per_block = SPMV_BLOCK / lpr;
+    const long rows_per_block = lpr == 1 ? spmv_tile<1>::RT : lpr <= 4 ? spmv_tile<4>::RT : SPMV_BLOCK / lpr;
     long grid = (rows + rows_per_block - 1) / rows_per_block;
     const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;
     if (grid > cap) grid = cap;

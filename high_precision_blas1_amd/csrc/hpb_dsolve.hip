@@ -750,13 +750,14 @@ __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict_
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) ds[q] = dc[q] = T(0);
     const int lane = threadIdx.x % LPR;
-    const dev::xcd_rows R = dev::xcd_rows::of(rows, DB / LPR);
-    for (long r = R.first + (long)threadIdx.x / LPR; r < R.end; r += R.stride) {
-        const int beg = rp[r], end = rp[r + 1];
+    // one row: its lanes accumulate (staged: from the tile in LDS), merge, and lane 0
+    // stores y_r and takes the dot products
+    auto row = [&](long r, int beg, int end, const T* sv, const T* sx) {
         const T rtr = (NQ >= 1 && lane == 0) ? rt[r] : T(0);   // issued with the row bounds
         T s = T(0), c = T(0);
         if (MODE == 1 && lane == 0) s = b[r];
-        dev::csr_row_acc<T, LPR, CROWS, MODE == 1>(beg, end, lane, col, val, x, s, c);
+        if (sv != nullptr) dev::csr_row_acc_staged<T, LPR, CROWS, MODE == 1>(beg, end, lane, sv, sx, s, c);
+        else dev::csr_row_acc<T, LPR, CROWS, MODE == 1>(beg, end, lane, col, val, x, s, c);
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1) {
             const T s2 = __shfl_xor(s, m, 64);
@@ -779,6 +780,23 @@ __global__ __launch_bounds__(DB) void k_ds_spmv(int rows, const int* __restrict_
                 else ds[1] = dev::fma_rn<T>(v, v, ds[1]);
             }
         }
+    };
+    if constexpr (LPR <= 4) {   // staged tiles of 128 rows, up to 8 entries per row on average
+        constexpr int RT = 128, CAP = 1024;
+        __shared__ dev::csr_stage<T, RT, CAP> S;
+        const dev::xcd_rows R = dev::xcd_rows::of(rows, RT);
+        for (long r0 = R.first; r0 < R.end; r0 += R.stride) {
+            const int nr = (int)((R.end - r0) < RT ? (R.end - r0) : RT);
+            const bool staged = dev::csr_stage_tile<T, RT, CAP, DB>(S, r0, nr, rp, col, val, x);
+            const int e0 = staged ? S.rp[0] : 0;
+            for (int rr = threadIdx.x / LPR; rr < nr; rr += DB / LPR)
+                row(r0 + rr, S.rp[rr] - e0, S.rp[rr + 1] - e0, staged ? S.v : nullptr, S.x);
+            __syncthreads();   // the next tile overwrites S
+        }
+    } else {
+        const dev::xcd_rows R = dev::xcd_rows::of(rows, DB / LPR);
+        for (long r = R.first + (long)threadIdx.x / LPR; r < R.end; r += R.stride)
+            row(r, rp[r], rp[r + 1], nullptr, nullptr);
     }
     if constexpr (NQ > 0) {
 #pragma unroll
@@ -1355,7 +1373,7 @@ private:
         hpb_handle_t h = P.h;
         const double mean = P.rows > 0 ? (double)P.nnz / P.rows : 1.0;
         const int lpr = mean <= 6.0 ? 2 : (mean <= 12.0 ? 4 : (mean <= 24.0 ? 8 : 16));   // as the library SpMV
-        const long per_block = DB / lpr;
+        const long per_block = lpr <= 4 ? 128 : DB / lpr;   // staged tiles of 128 rows up to 4 lanes per row
         long grid = (P.rows + per_block - 1) / per_block;
         const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;   // the gathers want many waves
         grid = std::max(1L, std::min(grid, cap));
