@@ -55,7 +55,31 @@ def test_gmres_small(handle, gs, high, dev):
     xd = spla.spsolve(Mh.tocsc(), b)
     assert np.max(np.abs(x - xd)) < 1e-9
     xo, ito, hist = K.gmres_mgs(Mh, b, np.zeros_like(b), 30, 1e-12, 3000)
-    assert abs(res["iters"] - ito) <= 30, (res["iters"], ito)
+    # without early exit every variant runs whole restart cycles; measured on MI355X,
+    # all eight variants stop after the same cycle as the restatement (120 iterations)
+    assert res["iters"] == ito, (res["iters"], ito)
+    A.free()
+
+
+@pytest.mark.parametrize("dev", [0, 1])
+def test_gmres_history_matches_restatement(handle, dev):
+    """The monitor's history of GMRES(30) (MGS, no early exit): one check per restart at
+    the same iteration numbers as the numpy restatement of the reference's loop, with
+    true relative residuals that agree to within 12 % (a factor 10^0.05) from 1 down to
+    1e-12: a wrong rotation, sign or update would change the convergence curve."""
+    Mh, A, b = _problem(handle, 24, 20, 18, S.CONVDIFF7)
+    say("gmres history", dev)
+    db = H.DeviceArray.from_numpy(b, handle)
+    dx = H.DeviceArray.from_numpy(np.zeros_like(b), handle)
+    p = S.params(solver=S.GMRES, restart=30, rel_tol=1e-12, max_iters=3000, high_precision=1, gram_schmidt=0,
+                 early_exit=0, device_resident=dev, save_history=1)
+    res = S.solve(handle, A, db, dx, p, history_cap=200)
+    xo, ito, hist = K.gmres_mgs(Mh, b, np.zeros_like(b), 30, 1e-12, 3000)
+    h = np.array(res["history"])
+    o = np.array(hist)
+    assert h.shape == o.shape, (h[:, 0], o[:, 0])
+    assert np.array_equal(h[:, 0], o[:, 0])
+    assert np.all(np.abs(np.log10(h[:, 1]) - np.log10(o[:, 1])) <= 0.05), np.c_[h, o]
     A.free()
 
 
