@@ -279,6 +279,65 @@ __global__ __launch_bounds__(XBLOCK) void k_exact(size_t n, const void* __restri
     if (threadIdx.x == 0) part[(size_t)XL * G + blockIdx.x] = sflag;
 }
 
+// ---- block-parallel carry and rounding of the digit sums (the last block of
+// k_exact_digits; XL <= blockDim.x, thread w owns word w)
+
+// Carry so that words 0..XL-2 lie in [0, 2^32) and the top word keeps the sign (the
+// unique normal form, as acc_carry): rounds in which every word passes its carry to
+// the next at once, until none is left (a carry moves one word per round).
+__device__ __forceinline__ void acc_carry_par(long long* d, long long* q)
+{
+    const int w = threadIdx.x;
+    for (int round = 0; round < 2 * XL; ++round) {
+        const long long c = (w < XL - 1) ? (d[w] >> 32) : 0;   // floor division
+        if (w < XL) q[w] = c;
+        __syncthreads();
+        if (w < XL - 1) d[w] -= c * 4294967296ll;
+        if (w > 0 && w < XL) d[w] += q[w - 1];
+        __syncthreads();
+        const int outside = (w < XL - 1) && (d[w] < 0 || d[w] >= 4294967296ll);
+        if (!__syncthreads_or(outside)) break;
+    }
+}
+
+// RN(value) of the carried words d (value = sum d_w 2^(32 w - 1074)), ties to even;
+// d is left unchanged (e, q, mag and sh are scratch in LDS).  Every thread returns it.
+__device__ double acc_round_par(const long long* d, long long* e, long long* q, unsigned* mag, int* sh)
+{
+    const int w = threadIdx.x;
+    const bool neg = d[XL - 1] < 0;
+    if (w < XL) e[w] = neg ? -d[w] : d[w];   // |value| once carried again
+    __syncthreads();
+    if (neg) acc_carry_par(e, q);
+    if (w < XL) mag[w] = (unsigned)(e[w] & 0xffffffffll);
+    if (w == 0) sh[0] = -1;
+    __syncthreads();
+    if (w < XL && mag[w] != 0u) atomicMax(&sh[0], w);
+    __syncthreads();
+    const int top = sh[0];
+    if (top < 0) return 0.0;
+    const int msb = top * 32 + (31 - __builtin_clz(mag[top]));
+    const int ulp = msb > 52 ? msb - 52 : 0;   // ulp 0: subnormal or exact, no rounding
+    // sticky: any bit below the half bit ulp - 1
+    const int sb = ulp > 0 ? ulp - 1 : 0, full = sb >> 5;
+    const int below = (w < full && mag[w] != 0u) ? 1 : 0;
+    const int sticky_words = __syncthreads_or(below);
+    if (msb - 1074 > 1023) return neg ? -__builtin_inf() : __builtin_inf();
+    const int a = ulp >> 5, s = ulp & 31, nb = msb - ulp + 1;   // nb <= 53
+    const unsigned long long w0 = mag[a], w1 = a + 1 < XL ? mag[a + 1] : 0u, w2 = a + 2 < XL ? mag[a + 2] : 0u;
+    const unsigned long long lo64 = w0 | (w1 << 32);
+    unsigned long long qv = s == 0 ? lo64 : ((lo64 >> s) | (w2 << (64 - s)));
+    qv &= (1ull << nb) - 1ull;
+    if (ulp > 0) {
+        const int half = (int)((mag[sb >> 5] >> (sb & 31)) & 1u);
+        int sticky = sticky_words;
+        if (!sticky && (sb & 31)) sticky = (mag[full] & ((1u << (sb & 31)) - 1u)) != 0u;
+        if (half && (sticky || (qv & 1ull))) qv += 1;
+    }
+    const double r = ldexp((double)qv, ulp - 1074);
+    return neg ? -r : r;
+}
+
 // Digit sums over the G block records, one block per word (word XL: the flags,
 // OR-ed).  Each carried digit is < 2^32 (the top word small and signed), so G <= 2^20
 // blocks cannot overflow int64; integer sums, so the order does not matter.  The
@@ -289,10 +348,12 @@ __global__ __launch_bounds__(256) void k_exact_digits(unsigned G, const long lon
                                                       long long* __restrict__ sums, unsigned* __restrict__ ticket,
                                                       double* __restrict__ out)
 {
+    static_assert(XL <= 256, "one word per thread");
     __shared__ long long red[256];
     __shared__ int am_last;
-    __shared__ long long acc[XL];
+    __shared__ long long acc[XL], e[XL], q[XL];
     __shared__ unsigned mag[XL];
+    __shared__ int sh[1];
     const unsigned w = blockIdx.x;
     const long long* p = part + (size_t)w * G;
     long long s = 0;
@@ -304,29 +365,36 @@ __global__ __launch_bounds__(256) void k_exact_digits(unsigned G, const long lon
                                                                      : (red[threadIdx.x] | red[threadIdx.x + m]);
         __syncthreads();
     }
-    if (threadIdx.x != 0) return;
-    sums[w] = red[0];
-    am_last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u;
+    if (threadIdx.x == 0) {
+        sums[w] = red[0];
+        am_last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u;
+    }
+    __syncthreads();
     if (!am_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int d = 0; d < XL; ++d) acc[d] = sums[d];
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < (unsigned)XL) acc[threadIdx.x] = sums[threadIdx.x];
     const int sflag = (int)sums[XL];
+    __syncthreads();
     double hi = __builtin_nan(""), lo = __builtin_nan("");
     if (sflag == 0) {
-        acc_carry(acc);
-        hi = acc_round(acc, mag);
+        acc_carry_par(acc, q);
+        hi = acc_round_par(acc, e, q, mag, sh);
         if (__builtin_isfinite(hi)) {
-            acc_add<false>(acc, -hi);
-            acc_carry(acc);
-            lo = acc_round(acc, mag);
+            __syncthreads();
+            if (threadIdx.x == 0) acc_add<false>(acc, -hi);
+            __syncthreads();
+            acc_carry_par(acc, q);
+            lo = acc_round_par(acc, e, q, mag, sh);
         } else {
             lo = 0.0;
         }
     }
-    out[0] = hi;
-    out[1] = lo;
-    out[2] = (double)sflag;
+    if (threadIdx.x == 0) {
+        out[0] = hi;
+        out[1] = lo;
+        out[2] = (double)sflag;
+    }
 }
 
 constexpr int XK = 6;   // terms per expansion (two expansions per thread)
