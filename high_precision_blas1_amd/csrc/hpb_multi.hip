@@ -38,8 +38,9 @@ struct coef_pack {
 // through the caches: the blocks of the other column chunks read the same w tiles
 // and find them in L2 (at n = 2^24, k = 64, streaming w as well cost 10 % of the
 // rate).  HOIST (one column chunk): the kc column loads of a tile are all issued
-// before the first Dot2 step; the order of the arithmetic is unchanged.
-template <typename T, int KC, bool COMP, int BLOCK, bool VEC, bool NT = false, bool HOIST = false>
+// before the first Dot2 step; UNR2 (with HOIST, k <= 4): two tiles' loads are issued
+// together.  The order of the arithmetic is unchanged by either.
+template <typename T, int KC, bool COMP, int BLOCK, bool VEC, bool NT = false, bool HOIST = false, bool UNR2 = false>
 __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T> V, const T* __restrict__ w,
                                                     T* __restrict__ part, const int* __restrict__ skip)
 {
@@ -57,38 +58,59 @@ __global__ __launch_bounds__(BLOCK) void k_multidot(size_t n, int k, ptr_pack<T>
         c[j] = T(0);
     }
     const Vt* wv = reinterpret_cast<const Vt*>(w);
-    for (size_t vi = (size_t)blockIdx.x * BLOCK + threadIdx.x; vi < nv; vi += (size_t)G * BLOCK) {
-        const Vt a = wv[vi];
-        Vt bh[HOIST ? KC : 1];
-        if constexpr (HOIST) {
+    auto load_col = [&](int j, size_t vi) -> Vt {
+        if constexpr (NT && VEC) return dev::load_nt<Vt>(reinterpret_cast<const Vt*>(V.p[j0 + j]) + vi);
+        else return reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
+    };
+    auto col_steps = [&](int j, const Vt& a, const Vt& b) {   // column j's Dot2 steps of one tile
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            T av, bv;
+            if constexpr (VEC) {
+                av = dev::vget<T>(a, q);
+                bv = dev::vget<T>(b, q);
+            } else {
+                av = a;
+                bv = b;
+            }
+            if (COMP) dev::dot2_step<T>(s[j], c[j], bv, av);
+            else s[j] = dev::fma_rn<T>(bv, av, s[j]);
+        }
+    };
+    size_t vi = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    const size_t vstride = (size_t)G * BLOCK;
+    if constexpr (HOIST && UNR2) {
+        for (; vi + vstride < nv; vi += 2 * vstride) {
+            const Vt a0 = wv[vi], a1 = wv[vi + vstride];
+            Vt b0[KC], b1[KC];
 #pragma unroll
             for (int j = 0; j < KC; ++j)
                 if (j < kc) {
-                    if constexpr (NT && VEC) bh[j] = dev::load_nt<Vt>(reinterpret_cast<const Vt*>(V.p[j0 + j]) + vi);
-                    else bh[j] = reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
+                    b0[j] = load_col(j, vi);
+                    b1[j] = load_col(j, vi + vstride);
                 }
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) col_steps(j, a0, b0[j]);
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) col_steps(j, a1, b1[j]);
         }
+    }
+    for (; vi < nv; vi += vstride) {
+        const Vt a = wv[vi];
+        if constexpr (HOIST) {
+            Vt bh[KC];
 #pragma unroll
-        for (int j = 0; j < KC; ++j) {
-            if (j < kc) {
-                Vt b;
-                if constexpr (HOIST) b = bh[j];
-                else if constexpr (NT && VEC) b = dev::load_nt<Vt>(reinterpret_cast<const Vt*>(V.p[j0 + j]) + vi);
-                else b = reinterpret_cast<const Vt*>(V.p[j0 + j])[vi];
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) bh[j] = load_col(j, vi);
 #pragma unroll
-                for (int q = 0; q < W; ++q) {
-                    T av, bv;
-                    if constexpr (VEC) {
-                        av = dev::vget<T>(a, q);
-                        bv = dev::vget<T>(b, q);
-                    } else {
-                        av = a;
-                        bv = b;
-                    }
-                    if (COMP) dev::dot2_step<T>(s[j], c[j], bv, av);
-                    else s[j] = dev::fma_rn<T>(bv, av, s[j]);
-                }
-            }
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) col_steps(j, a, bh[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+                if (j < kc) col_steps(j, a, load_col(j, vi));
         }
     }
     if constexpr (VEC) {
@@ -239,7 +261,10 @@ static int multidot_launch(hipStream_t st, int ncu, const hpb_tuning_t& tu, int 
 #define HPB_MDOT(KC_, C_, V_, NT_)                                                                               \
     do {                                                                                                        \
         if constexpr (V_) {                                                                                     \
-            if (nch == 1)                                                                                       \
+            if (k <= 4)                                                                                         \
+                hipLaunchKernelGGL((k_multidot<T, 4, C_, 256, V_, NT_, true, true>), grid, dim3(256), 0, st, n, k, \
+                                   pk, w, part, skip);                                                          \
+            else if (nch == 1)                                                                                  \
                 hipLaunchKernelGGL((k_multidot<T, KC_, C_, 256, V_, NT_, true>), grid, dim3(256), 0, st, n, k, pk, \
                                    w, part, skip);                                                              \
             else                                                                                                \
