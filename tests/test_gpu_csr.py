@@ -136,7 +136,19 @@ def test_spmv_bitexact(handle, lpr, comp):
     M.data = r.standard_normal(M.nnz) * np.exp2(r.integers(-20, 20, M.nnz))
     M = M.tocsr()
     M.sort_indices()
-    cases = [M, O.stencil7(12, 10, 9, CONVDIFF)]
+    # staged tiles (up to 4 lanes per row): a bidiagonal matrix (about 2 entries per
+    # row: one lane's tiles of 256 rows fit), and rows of uneven length with empty
+    # rows, so that some 128-row tiles are staged and others read directly
+    B2 = sp.diags([r.standard_normal(2500), r.standard_normal(2499)], [0, -1], format="csr")
+    lens = np.where(np.arange(3000) < 700, 20, r.integers(0, 4, 3000))
+    U = sp.lil_matrix((3000, 3000))
+    for i, ln in enumerate(lens):
+        cols = r.choice(3000, size=int(ln), replace=False)
+        U[i, cols] = r.standard_normal(int(ln))
+    U = U.tocsr()
+    for Z in (B2, U):
+        Z.sort_indices()
+    cases = [M, O.stencil7(12, 10, 9, CONVDIFF), B2, U]
     for Mh in cases:
         say("spmv", lpr, comp, Mh.shape, Mh.nnz)
         A = upload_csr(handle, Mh)
@@ -186,3 +198,32 @@ def test_poisson_256_cubed_properties(handle):
     y2 = spmv(handle, A, x, comp=True).numpy()
     assert np.array_equal(y2, 2.0 * y1)
     A.free()
+
+
+@pytest.mark.parametrize("comp", [False, True])
+def test_spmv_f32_integer_exact(handle, comp):
+    """FP32 rows (the staged tiles of the mixed-precision inner solves) on integer data:
+    every product and partial sum is exact in FP32, so y must equal the integer
+    product whatever the order; a dropped, repeated or misplaced entry shows."""
+    r = np.random.default_rng(17 + comp)
+    cases = [O.stencil7(12, 10, 9, POISSON)]
+    M = sp.random(3000, 3000, density=0.003, format="csr", random_state=3)
+    M.data = r.integers(-8, 9, M.nnz).astype(np.float64)
+    cases.append(M)
+    for Mh in cases:
+        Mh = Mh.tocsr()
+        Mh.sort_indices()
+        say("spmv f32 integer", comp, Mh.shape, Mh.nnz)
+        rows = Mh.shape[0]
+        x = r.integers(-8, 9, rows).astype(np.float32)
+        rp = H.DeviceArray.from_numpy(Mh.indptr.astype(np.int32), handle)
+        col = H.DeviceArray.from_numpy(Mh.indices.astype(np.int32), handle)
+        val = H.DeviceArray.from_numpy(Mh.data.astype(np.float32), handle)
+        dx = H.DeviceArray.from_numpy(x, handle)
+        dy = H.DeviceArray(rows, np.float32, handle)
+        H.call("hpb_csr_spmvc_f32" if comp else "hpb_csr_spmv_f32", handle.ptr, rows, rows, Mh.nnz, rp.ptr, col.ptr,
+               val.ptr, ctypes.c_float(1.0), dx.ptr, ctypes.c_float(0.0), dy.ptr)
+        exact = (Mh.astype(np.int64) @ x.astype(np.int64)).astype(np.float32)
+        assert np.array_equal(dy.numpy(), exact)
+        for a in (rp, col, val, dx, dy):
+            a.free()
