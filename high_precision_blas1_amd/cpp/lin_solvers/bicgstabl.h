@@ -14,7 +14,10 @@
 // preconditioner requires use_precond_resid.  The shadow vector is r_0 / ||r_0||.
 // The MR coefficients tau_ij are kept in an (L+1) x (L+1) table indexed 1..L.
 // resid_recalc_freq = k > 0 recomputes r_0 = P (b - A x) after every k-th cycle.
-// A vanishing rho in the BiCG part ends the solve; breakdown() reports it.  With
+// A vanishing denominator (rho or (A u_j, r~) in the BiCG part, (r_j, r_j) in the MR
+// part) ends the solve before x changes; breakdown() reports it unless the iterate's
+// true residual then meets the tolerance (a lucky breakdown: the residual vanished,
+// as on the identity), which counts as convergence.  With
 // the monitor's out_min_resid_norm option the iterate of smallest checked residual
 // is copied back into x before returning.
 #pragma once
@@ -110,8 +113,10 @@ public:
         if (rt_norm != T(0)) vec_ops_->scale(T(1) / rt_norm, rtilde_);
 
         T rho = T(1), alpha = T(0), omega = T(1);
+        auto vanished = [](T d) { return d == T(0) || !(d == d); };
         while (!monitor_.check_finished(x, r_[0])) {
             rho = -omega * rho;
+            int steps = 0;   // BiCG steps of this cycle that updated x
             for (int j = 0; j < L; ++j) {
                 if (rho == T(0)) {
                     breakdown_ = true;
@@ -123,25 +128,43 @@ public:
                 for (int i = 0; i <= j; ++i) vec_ops_->add_mul(T(1), r_[i], -beta, u_[i]);  // u_i = r_i - beta u_i
                 A.apply(u_[j], u_[j + 1]);
                 if (prec_ != nullptr) prec_->apply(u_[j + 1]);
-                alpha = rho / vec_ops_->scalar_prod(u_[j + 1], rtilde_);
+                const T sg = vec_ops_->scalar_prod(u_[j + 1], rtilde_);
+                if (vanished(sg)) {   // stop before x changes
+                    breakdown_ = true;
+                    break;
+                }
+                alpha = rho / sg;
                 for (int i = 0; i <= j; ++i) vec_ops_->add_mul(-alpha, u_[i + 1], T(1), r_[i]);  // r_i -= alpha u_{i+1}
                 A.apply(r_[j], r_[j + 1]);
                 if (prec_ != nullptr) prec_->apply(r_[j + 1]);
                 vec_ops_->add_mul(alpha, u_[0], T(1), x);  // x += alpha u_0
+                steps = j + 1;
             }
             if (breakdown_) {
-                logged_obj_t::error_f("solve: breakdown at iteration %d: (r_j, r~) vanished in the BiCG part",
-                                      monitor_.iters_performed());
+                logged_obj_t::info_f("solve: the BiCG part stopped at iteration %d: (r_j, r~) or (A u_j, r~) vanished",
+                                     monitor_.iters_performed() + steps);
+                monitor_ += steps;
                 break;
             }
-            // MR part: modified Gram-Schmidt on r_1..r_L
-            for (int j = 1; j <= L; ++j) {
+            // MR part: modified Gram-Schmidt on r_1..r_L; a vanishing sigma_j (r_j = 0)
+            // ends the solve before the MR update touches x
+            for (int j = 1; j <= L && !breakdown_; ++j) {
                 for (int i = 1; i < j; ++i) {
                     TAU(i, j) = vec_ops_->scalar_prod(r_[j], r_[i]) / sigma[i];
                     vec_ops_->add_mul(-TAU(i, j), r_[i], T(1), r_[j]);
                 }
                 sigma[j] = vec_ops_->scalar_prod(r_[j], r_[j]);
+                if (vanished(sigma[j])) {
+                    breakdown_ = true;
+                    break;
+                }
                 gamma_p[j] = vec_ops_->scalar_prod(r_[0], r_[j]) / sigma[j];
+            }
+            if (breakdown_) {
+                logged_obj_t::info_f("solve: the MR part stopped at iteration %d: (r_j, r_j) vanished",
+                                     monitor_.iters_performed() + L);
+                monitor_ += L;
+                break;
             }
             gamma[L] = gamma_p[L];
             omega = gamma[L];
@@ -171,6 +194,15 @@ public:
                 calc_residual_(A, x, b, r_[0]);
             }
             monitor_ += L;
+        }
+        if (breakdown_) {
+            // a vanishing denominator ended the iteration; when the iterate already solves
+            // the system (its residual vanished: a lucky breakdown) that is convergence
+            calc_residual_(A, x, b, r_[0]);
+            if (use_precond_resid_ && prec_ != nullptr) prec_->apply(r_[0]);
+            monitor_.check_finished(x, r_[0]);
+            if (monitor_.converged()) breakdown_ = false;
+            else logged_obj_t::error_f("solve: breakdown at iteration %d", monitor_.iters_performed());
         }
         if (monitor_.out_min_resid_norm()) vec_ops_->assign(monitor_.min_resid_norm_x(), x);
         const bool ok = monitor_.converged() && !breakdown_;

@@ -34,6 +34,27 @@ struct system_operator {
     }
 };
 
+// the identity: BiCGStab(L) reaches the exact solution in its first BiCG step, after
+// which every denominator vanishes (a lucky breakdown)
+template <class VO>
+struct identity_operator {
+    using T = typename VO::scalar_type;
+    using T_vec = typename VO::vector_type;
+    int sz;
+    explicit identity_operator(const VO* vo) : sz((int)vo->sz_) {}
+    void apply(const T_vec& x, T_vec& f) const
+    {
+        for (int i = 0; i < sz; ++i) f[i] = x[i];
+    }
+};
+
+template <class VO, class Op>
+struct null_prec {   // the solver's preconditioner type; never set
+    using T_vec = typename VO::vector_type;
+    void set_operator(const Op*) {}
+    void apply(T_vec&) const {}
+};
+
 template <class VO, class Op>
 struct prec_operator {
     using T = typename VO::scalar_type;
@@ -151,6 +172,22 @@ static void run_all(const char* tag, bool high)
                 std::snprintf(name, sizeof name, "%s_bicgstab_prec%d_pr%d_rf%d", tag, pc, pr, rf);
                 report(name, vo, A, ok && !s.breakdown(), s.monitor().iters_performed(), x, b, xd);
             }
+    // lucky breakdown on the identity: x = b exactly, reported as converged
+    //   name converged iterations max|x - b| breakdown
+    for (int L : {1, 2, 4}) {
+        typedef identity_operator<VO> id_t;
+        typedef null_prec<VO, id_t> idp_t;
+        id_t I(&vo);
+        bicgstabl<id_t, idp_t, VO, mon_t, hpb::log_std> s(&vo);
+        s.set_basis_size(L);
+        s.monitor().init(1e-12, 0.0, 100);
+        vo.assign_scalar(0.0, x);
+        const bool ok = s.solve(I, b, x);
+        double err = 0;
+        for (int i = 0; i < n; ++i) err = std::max(err, std::fabs(x[i] - b[i]));
+        std::printf("%s_identity_bicgstabl%d %d %d %.6e %d\n", tag, L, ok ? 1 : 0, s.monitor().iters_performed(), err,
+                    s.breakdown() ? 1 : 0);
+    }
     // failure handling: iteration cap, NaN in the right-hand side
     {
         gmres<op_t, prec_t, VO, mon_t, hpb::log_std, 100> s(&vo);

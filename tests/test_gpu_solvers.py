@@ -228,3 +228,39 @@ def test_mixed_precision_badly_scaled_operator(handle, L, refinement):
     assert res["iters"] > 0
     assert _host_rel(Mh, b, dx.numpy()) <= 1.1e-12
     A.free()
+
+
+@pytest.mark.parametrize("dev", [0, 1])
+def test_textbook_special_cases(handle, dev):
+    """GMRES on a diagonal matrix with k distinct eigenvalues reaches the solution in k
+    steps (the minimal polynomial has degree k); BiCGStab(L) on the identity converges
+    in its first cycle; both to the true residual they report."""
+    import scipy.sparse as sp
+    n = 20000
+    r = np.random.default_rng(11)
+    for k in (1, 3, 5):
+        eig = np.array([1.0, 2.5, 4.0, 7.0, 11.0])[:k]
+        D = sp.diags(eig[r.integers(0, k, n)], 0, format="csr")
+        A = S.DeviceCSR.from_scipy(handle, D)
+        b = r.uniform(-1, 1, n)
+        db = H.DeviceArray.from_numpy(b, handle)
+        dx = H.DeviceArray.from_numpy(np.zeros(n), handle)
+        say("gmres diagonal", k, dev)
+        p = S.params(solver=S.GMRES, restart=30, rel_tol=1e-12, max_iters=100, high_precision=1, early_exit=1,
+                     device_resident=dev)
+        res = S.solve(handle, A, db, dx, p)
+        assert res["converged"] == 1 and res["iters"] == k, (k, res["iters"])
+        assert _host_rel(D, b, dx.numpy()) <= 1e-12
+        A.free()
+    I = sp.identity(n, format="csr")
+    A = S.DeviceCSR.from_scipy(handle, I)
+    b = r.uniform(-1, 1, n)
+    for L in (1, 2, 4):
+        db = H.DeviceArray.from_numpy(b, handle)
+        dx = H.DeviceArray.from_numpy(np.zeros(n), handle)
+        say("bicgstabl identity", L, dev)
+        p = S.params(solver=S.BICGSTABL, basis=L, rel_tol=1e-12, max_iters=100, high_precision=1, device_resident=dev)
+        res = S.solve(handle, A, db, dx, p)
+        assert res["converged"] == 1 and res["iters"] <= L, (L, res["iters"])
+        assert np.max(np.abs(dx.numpy() - b)) <= 1e-15
+    A.free()

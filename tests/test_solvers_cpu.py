@@ -31,7 +31,8 @@ def results(tmp_path_factory):
 
 
 def test_all_variants_converge(results):
-    solves = {k: v for k, v in results.items() if ("gmres" in k or "bicgstab" in k) and "_minres_" not in k}
+    solves = {k: v for k, v in results.items()
+              if ("gmres" in k or "bicgstab" in k) and "_minres_" not in k and "_identity_" not in k}
     assert len(solves) == 21 + 24 + 6
     for name, (conv, iters, rel, err) in solves.items():
         assert conv == "1", name
@@ -68,3 +69,14 @@ def test_min_resid_norm_iterate_returned(results):
             mn, tr, last = (float(v) for v in results["%s_minres_%s" % (tag, solver)])
             assert mn <= last * (1 + 1e-12), (tag, solver, mn, last)
             assert abs(tr - mn) <= 1e-6 * mn, (tag, solver, tr, mn)
+
+
+def test_lucky_breakdown_on_identity(results):
+    # BiCGStab(L) on the identity: the first BiCG step gives x = b exactly, then the
+    # denominators vanish; the solver stops without touching x and, the true residual
+    # being zero, reports convergence rather than a breakdown
+    for tag in ("plain", "high", "multi"):
+        for L in (1, 2, 4):
+            conv, iters, err, brk = results["%s_identity_bicgstabl%d" % (tag, L)]
+            assert conv == "1" and brk == "0", (tag, L, conv, brk)
+            assert 1 <= int(iters) <= L and float(err) == 0.0, (tag, L, iters, err)
