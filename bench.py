@@ -299,7 +299,7 @@ def cfg_spmv(H, h, log):
     res = {"rows": A.rows, "nnz": A.nnz, "algorithmic_bytes": int(nbytes)}
     for name, fn in (("plain", L.hpb_csr_spmv_f64), ("compensated", L.hpb_csr_spmvc_f64)):
         ms = timed(h, fn, (h.ptr, A.rows, A.rows, A.nnz, A.rp.ptr, A.col.ptr, A.val.ptr, ctypes.c_double(1.0), xs.ptr,
-                           ctypes.c_double(0.0), ys.ptr), 20, 3)
+                           ctypes.c_double(0.0), ys.ptr), 50, 20)
         res[name + "_gbs"] = round(nbytes / ms / 1e6, 1)
         res[name + "_ms"] = round(ms, 4)
     A.free()

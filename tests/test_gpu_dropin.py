@@ -57,7 +57,7 @@ def _matrix(m, seed):
 def _lpr(M):
     """The library's lanes per row: about two entries per lane."""
     mean = M.nnz / M.shape[0]
-    for lim, lanes in ((1.5, 1), (6, 2), (12, 4), (24, 8), (48, 16), (96, 32)):
+    for lim, lanes in ((1.5, 1), (8, 2), (12, 4), (24, 8), (48, 16), (96, 32)):
         if mean <= lim:
             return lanes
     return 64
