@@ -13,6 +13,8 @@
 // that every y_i is accurate to ~u|row.x| + gamma^2 |row|.|x|.
 // The residual kernel forms r = b - A x in the same way with b_i entered as the
 // first term of the compensated sum (used by iterative refinement).
+// Rows of up to 4 lanes are formed from staged tiles (csr_stage_tile) whose columns
+// and values are loaded nontemporally; y is then stored nontemporally too (NT).
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -31,7 +33,7 @@ constexpr int SPMV_BLOCK = 256;
 template <int LPR> struct spmv_tile { static constexpr int RT = LPR == 1 ? 256 : 128, CAP = LPR == 1 ? 512 : 1024; };
 template <int LPR> constexpr bool spmv_staged() { return LPR <= 4; }
 
-template <typename T, int LPR, bool COMP, int MODE>
+template <typename T, int LPR, bool COMP, int MODE, bool NT>
 __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __restrict__ rp,
                                                      const int* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ x, T alpha, T beta,
@@ -42,8 +44,6 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
     if (skip != nullptr && *skip) return;   // solver graphs: flag set, launch does nothing
     if (dalpha != nullptr) alpha = *dalpha;  // solver graphs: scale factor from device memory
     const int lane = threadIdx.x % LPR;
-    // staged tiles stream y out nontemporally, as they stream the columns and values in
-    constexpr bool NT = spmv_staged<LPR>();
     auto finish = [&](long r, T s, T c) {
 #pragma unroll
         for (int m = LPR / 2; m >= 1; m >>= 1) {
@@ -96,17 +96,19 @@ __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv(int rows, const int* __rest
     }
 }
 
+// Staged tiles stream the columns and values in and y out nontemporally (NT).
 template <typename T, bool COMP, int MODE, int LPR>
 static void spmv_launch_l(hpb_handle_t h, int grid, int rows, const int* rp, const int* col, const T* val,
                           const T* x, T alpha, T beta, const T* b, T* y, const int* skip, const T* dalpha)
 {
-    hipLaunchKernelGGL((k_spmv<T, LPR, COMP, MODE>), dim3(grid), dim3(SPMV_BLOCK), 0, h->stream, rows, rp, col, val, x,
-                       alpha, beta, b, y, skip, dalpha);
+    hipLaunchKernelGGL((k_spmv<T, LPR, COMP, MODE, spmv_staged<LPR>()>), dim3(grid), dim3(SPMV_BLOCK), 0, h->stream,
+                       rows, rp, col, val, x, alpha, beta, b, y, skip, dalpha);
 }
 
 // Rows of up to 8 entries on average fill the staged tiles (128 rows, 1024 entries)
 // and take 2 lanes: 256^3 7-point FP64, 5271 GB/s with 2 lanes, 5156 with 4, 3397
-// with 8 (direct rows); 4096^2 5-point, 5246 with 2, 5045 with 4, 3689 with 1.
+// with 8 (direct rows); 4096^2 5-point, 5246 with 2, 5045 with 4, 3689 with 1
+// (128 blocks per CU).
 // Longer rows read their entries directly, about two entries per lane.
 static int pick_lpr(long rows, long nnz)
 {
@@ -136,9 +138,11 @@ static int spmv_entry(hpb_handle_t h, int rows, int cols, long nnz, const int* r
                 HPB_ERR_INVALID, "csr spmv: lanes per row must be a power of two <= 64");
     const long rows_per_block = lpr == 1 ? spmv_tile<1>::RT : lpr <= 4 ? spmv_tile<4>::RT : SPMV_BLOCK / lpr;
     long grid = (rows + rows_per_block - 1) / rows_per_block;
-    // staged tiles: up to 128 blocks per CU (a block's few tiles, launched in XCD
-    // order, keep the gathered planes in that XCD's L2); direct rows: 16 per CU
-    const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * (lpr <= 4 ? 128 : 16);
+    // staged tiles: up to 256 blocks per CU, so a block forms a few tiles, strided
+    // over its XCD's slice; 256^3 7-point: 5455 GB/s, against 5334 with one tile
+    // per block, 5265 with 128 blocks per CU and 4918 with 16 (HBM reads 1.08, 1.01,
+    // 1.18 and 1.02 x the algorithmic bytes).  Direct rows: 16 blocks per CU.
+    const long cap = (long)(h->num_cu > 0 ? h->num_cu : 256) * (lpr <= 4 ? 256 : 16);
     if (grid > cap) grid = cap;
     if (grid < 1) grid = 1;
     switch (lpr) {

@@ -237,7 +237,7 @@ __device__ __forceinline__ void csr_row_acc(int beg, int end, int lane, const in
 // per row, RT 128, CAP 1024) the SpMV takes 378 us against 410 us direct: a lane's
 // strided 4- and 8-byte loads of its own row entries were the cost, not the gather.
 // Streaming the columns and values in and y out nontemporally, with a few tiles per
-// block (2 lanes per row), takes it to 330 us.
+// block (2 lanes per row), takes it to 320-330 us.
 template <typename T, int RT, int CAP>
 struct csr_stage {
     T v[CAP];
