@@ -1128,11 +1128,12 @@ int hpb_batch_create_diffusion2d(hpb_handle_t h, int nsys, const int* sides, dou
     }
     {
         const int cu = h->num_cu > 0 ? h->num_cu : 256;
-        // measured on 512 systems (186.6M rows): the row-product kernels run best with
-        // 8 blocks per CU (4 equal, 2 half as fast); the streaming updates with 1
-        // (8 to 12 % faster than 8, as the library's tuned elementwise shape has few
-        // blocks per CU)
-        B->grid = std::min(B->ntiles, 8 * cu);
+        // measured on 512 systems (186.6M rows, 91 368 tiles): the row-product kernels
+        // run best with one tile per block (whole solve 18.59 s, against 18.71 s at 128
+        // blocks per CU, 18.92 s at 32 and 19.86 s at 8; with the nontemporal staging);
+        // the streaming updates with 1 block per CU (18.61 s, against 19.11 s at 4,
+        // 19.15 s at 32 and 18.72 s with one tile per block)
+        B->grid = (int)std::min<int64_t>(B->ntiles, (int64_t)512 * cu);
         B->ugrid = std::min(B->ntiles, cu);
     }
     HPB_HIP(hipStreamSynchronize(st));

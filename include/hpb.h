@@ -475,7 +475,7 @@ int hpb_batch_info(hpb_batch_t b, int64_t* rows, int64_t* nnz, int* ntiles);
  * per CU) whose phases are separated by grid barriers and which stops by itself when
  * no system is active (check_every unused); the three modes give the same iterates
  * bit for bit.  On MI355X the graph replay is the fastest (the 512-system solve:
- * 20.1 s replayed, 21.1 s persistent).  Every kernel of a cycle runs on a fixed grid that
+ * 18.6 s replayed, 21.2 s persistent).  Every kernel of a cycle runs on a fixed grid that
  * strides over the list of tiles of systems still active (rebuilt on the device at
  * the end of each cycle), so converged systems cost nothing.  Per-system outputs (host, nsys entries, each
  * optional): iterations, true relative residual, converged flag; plus wall seconds
