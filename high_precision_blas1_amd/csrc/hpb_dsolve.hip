@@ -280,8 +280,10 @@ __device__ __forceinline__ void merge_columns(const T* __restrict__ part, unsign
 
 // One Gram-Schmidt projection pass for column j: per-block partials of
 // (V_i, w), i <= j, compensated or plain, KC columns per blockIdx.y chunk;
-// k_gm_hfin merges them.
-template <typename T, int KC, bool COMP>
+// k_gm_hfin merges them.  NT: the basis columns are loaded nontemporally (chosen
+// when the basis is larger than the 256 MB MALL, so it cannot stay there between
+// passes anyway).
+template <typename T, int KC, bool COMP, bool NT>
 __global__ __launch_bounds__(DB) void k_gm_dot(size_t n, int k, basis_pack<T> V, const T* __restrict__ w,
                                                T* __restrict__ part, const int* __restrict__ stop)
 {
@@ -301,7 +303,8 @@ __global__ __launch_bounds__(DB) void k_gm_dot(size_t n, int k, basis_pack<T> V,
 #pragma unroll
         for (int q = 0; q < KC; ++q)
             if (q < kc) {
-                const Vt v = reinterpret_cast<const Vt*>(V.p[j0 + q])[i];
+                const Vt* col = reinterpret_cast<const Vt*>(V.p[j0 + q]);
+                const Vt v = NT ? dev::load_nt<Vt>(col + i) : col[i];
 #pragma unroll
                 for (int e = 0; e < W; ++e) {
                     if (COMP) dev::dot2_step<T>(s[q], c[q], dev::vget<T>(v, e), dev::vget<T>(a, e));
@@ -359,8 +362,9 @@ __global__ __launch_bounds__(1024) void k_gm_hfin(const T* __restrict__ part, un
 }
 
 // w += sum_{i<k} coef_i V_i (Dot2 over the terms of each element when COMP);
-// NORM: also per-block partials of ||w||^2 (compensated) for k_gm_nfin.
-template <typename T, bool COMP, bool NORM>
+// NORM: also per-block partials of ||w||^2 (compensated) for k_gm_nfin.  NT as in
+// k_gm_dot.
+template <typename T, bool COMP, bool NORM, bool NT>
 __global__ __launch_bounds__(DB) void k_gm_upd(size_t n, int k, basis_pack<T> V, const T* __restrict__ coef,
                                                T* __restrict__ w, T* __restrict__ part,
                                                const int* __restrict__ skip)
@@ -383,7 +387,8 @@ __global__ __launch_bounds__(DB) void k_gm_upd(size_t n, int k, basis_pack<T> V,
             c[e] = T(0);
         }
         for (int q = 0; q < k; ++q) {
-            const Vt v = reinterpret_cast<const Vt*>(V.p[q])[i];
+            const Vt* col = reinterpret_cast<const Vt*>(V.p[q]);
+            const Vt v = NT ? dev::load_nt<Vt>(col + i) : col[i];
             const T cq = sc[q];
 #pragma unroll
             for (int e = 0; e < W; ++e) {
@@ -1281,6 +1286,11 @@ private:
         // grids over 16-byte vectors; the reductions end in per-block partials
         // merged by one-block kernels (k_gm_hfin, k_gm_nfin)
         const size_t gvec = (n / dev::vec16<T>::W + DB - 1) / DB;
+        // basis larger than the 256 MB MALL: stream it (128^3 FP64 GMRES(50), 816 MB:
+        // 0.275 s against 0.293 s; 256^3 FP64 GMRES(30): 11.79 s against 12.63 s); a
+        // basis that fits is read back from the MALL by the next pass (128^3 FP32
+        // GMRES(30), 248 MB: 0.106 s streamed against 0.098 s)
+        const bool nt_basis = (double)(m + 1) * (double)n * sizeof(T) > 256.0 * 1024 * 1024;
         const unsigned gupd = (unsigned)std::max<size_t>(1, std::min(gvec, 8 * cus));
         constexpr int KC = 8;
         HPB_HIP(hipStreamSynchronize(st));
@@ -1302,21 +1312,21 @@ private:
                     const unsigned nch = (unsigned)((kk + KC - 1) / KC);
                     size_t gd = std::min(gvec, std::max<size_t>(cus, (4 * cus) / nch));
                     gd = std::max<size_t>(1, std::min(gd, part_pairs / (size_t)kk));
-                    if (comp) {
-                        hipLaunchKernelGGL((k_gm_dot<T, KC, true>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, kk,
-                                           bp, V[j + 1], part, skip);
-                        hipLaunchKernelGGL((k_gm_hfin<T, true>), dim3(1), dim3(1024), 0, st, part, (unsigned)gd, k,
-                                           kk, sig, Hj, coef.p, pass, skip, S);
-                    } else {
-                        hipLaunchKernelGGL((k_gm_dot<T, KC, false>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n, kk,
-                                           bp, V[j + 1], part, skip);
-                        hipLaunchKernelGGL((k_gm_hfin<T, false>), dim3(1), dim3(1024), 0, st, part, (unsigned)gd, k,
-                                           kk, sig, Hj, coef.p, pass, skip, S);
-                    }
+                    auto dot = [&](auto compc, auto ntc) {
+                        constexpr bool C = decltype(compc)::value, NTB = decltype(ntc)::value;
+                        hipLaunchKernelGGL((k_gm_dot<T, KC, C, NTB>), dim3((unsigned)gd, nch), dim3(DB), 0, st, n,
+                                           kk, bp, V[j + 1], part, skip);
+                        hipLaunchKernelGGL((k_gm_hfin<T, C>), dim3(1), dim3(1024), 0, st, part, (unsigned)gd, k, kk,
+                                           sig, Hj, coef.p, pass, skip, S);
+                    };
                     auto upd = [&](auto compc, auto normc) {
                         constexpr bool C = decltype(compc)::value, NM = decltype(normc)::value;
-                        hipLaunchKernelGGL((k_gm_upd<T, C, NM>), dim3(gupd), dim3(DB), 0, st, n, k, bp, coef.p,
-                                           V[j + 1], part, skip);
+                        if (nt_basis)
+                            hipLaunchKernelGGL((k_gm_upd<T, C, NM, true>), dim3(gupd), dim3(DB), 0, st, n, k, bp,
+                                               coef.p, V[j + 1], part, skip);
+                        else
+                            hipLaunchKernelGGL((k_gm_upd<T, C, NM, false>), dim3(gupd), dim3(DB), 0, st, n, k, bp,
+                                               coef.p, V[j + 1], part, skip);
                         if (NM && dgks && pass == 0)
                             hipLaunchKernelGGL((k_gm_dgks<T, C>), dim3(1), dim3(DB), 0, st, part, gupd, j, m, H.p,
                                                ldh, cs.p, sn.p, g.p, sig, S);
@@ -1326,6 +1336,13 @@ private:
                     };
                     using BT = std::true_type;
                     using BF = std::false_type;
+                    if (comp) {
+                        if (nt_basis) dot(BT{}, BT{});
+                        else dot(BT{}, BF{});
+                    } else {
+                        if (nt_basis) dot(BF{}, BT{});
+                        else dot(BF{}, BF{});
+                    }
                     const bool norm = pass == 1 || dgks;
                     if (comp) {
                         if (norm) upd(BT{}, BT{});
