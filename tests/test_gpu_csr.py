@@ -200,6 +200,28 @@ def test_poisson_256_cubed_properties(handle):
     A.free()
 
 
+def test_spmv_256_cubed_bitexact(handle):
+    """At 256^3 the staged kernel's grid is capped (256 blocks per CU), so blocks form
+    several 128-row tiles strided over their XCD's slice: every row still equals the
+    oracle's row-order emulation, bit for bit, plain and compensated, 2 and 4 lanes."""
+    from types import SimpleNamespace
+    nx = 256
+    say("spmv 256^3 bit-exact")
+    A = build_stencil(handle, nx, nx, nx, CONVDIFF)
+    n = A.rows
+    Mh = SimpleNamespace(indptr=A.rp.numpy(), indices=A.col.numpy(), data=A.val.numpy(), shape=(n, n))
+    x = H.DeviceArray(n, np.float64, handle)
+    H.call("hpb_random_uniform_f64", handle.ptr, n, 5, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), x.ptr)
+    xh = x.numpy()
+    for lpr in (0, 4):
+        for comp in (False, True):
+            say("spmv 256^3", lpr, comp)
+            y = spmv(handle, A, x, lpr=lpr, comp=comp).numpy()
+            ref = O.csr_spmv(Mh, xh, lpr=lpr if lpr else 2, comp=comp)
+            assert y.tobytes() == ref.tobytes()
+    A.free()
+
+
 @pytest.mark.parametrize("comp", [False, True])
 def test_spmv_f32_integer_exact(handle, comp):
     """FP32 rows (the staged tiles of the mixed-precision inner solves) on integer data:
