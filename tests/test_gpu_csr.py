@@ -222,6 +222,36 @@ def test_spmv_256_cubed_bitexact(handle):
     A.free()
 
 
+def test_spmv_f32_256_cubed_integer_exact(handle):
+    """FP32 rows at 256^3 (tiles of 256 rows, several per block): Poisson values and
+    integer x are exact in FP32, so y must equal the integer stencil product."""
+    nx = 256
+    say("spmv f32 256^3 integer")
+    A = build_stencil(handle, nx, nx, nx, POISSON)
+    n = A.rows
+    valf = H.DeviceArray(A.nnz, np.float32, handle)
+    H.call("hpb_convert_f64_to_f32", handle.ptr, A.nnz, A.val.ptr, valf.ptr)
+    r = np.random.default_rng(31)
+    x = r.integers(-8, 9, n).astype(np.float32)
+    g = x.reshape(nx, nx, nx).astype(np.int64)   # [k, j, i]
+    exact = 6 * g
+    exact[1:] -= g[:-1]
+    exact[:-1] -= g[1:]
+    exact[:, 1:] -= g[:, :-1]
+    exact[:, :-1] -= g[:, 1:]
+    exact[:, :, 1:] -= g[:, :, :-1]
+    exact[:, :, :-1] -= g[:, :, 1:]
+    dx = H.DeviceArray.from_numpy(x, handle)
+    dy = H.DeviceArray(n, np.float32, handle)
+    for name in ("hpb_csr_spmv_f32", "hpb_csr_spmvc_f32"):
+        H.call(name, handle.ptr, n, n, A.nnz, A.rp.ptr, A.col.ptr, valf.ptr, ctypes.c_float(1.0), dx.ptr,
+               ctypes.c_float(0.0), dy.ptr)
+        assert np.array_equal(dy.numpy(), exact.reshape(-1).astype(np.float32)), name
+    for a in (valf, dx, dy):
+        a.free()
+    A.free()
+
+
 @pytest.mark.parametrize("comp", [False, True])
 def test_spmv_f32_integer_exact(handle, comp):
     """FP32 rows (the staged tiles of the mixed-precision inner solves) on integer data:
