@@ -519,7 +519,7 @@ def cfg_multidot(H, h, log, ks=None):
     rows = []
     for k in ks or (1, 2, 4, 8, 16, 32, 50, 64):
         ptrs = (ctypes.c_void_p * k)(*[a.addr for a in V[:k]])
-        ms = timed(h, L.hpb_multidot_f64_async, (h.ptr, k, ptrs, nmax, w.ptr, 1, d.ptr), 10, 2)
+        ms = timed(h, L.hpb_multidot_f64_async, (h.ptr, k, ptrs, nmax, w.ptr, 1, d.ptr), 50, 10)
         nbytes = 8.0 * nmax * (k + 1)
         rows.append({"k": k, "ms": round(ms, 4), "gbs_algorithmic": round(nbytes / ms / 1e6, 1),
                      "separate_dot2_bytes_ratio": round(2.0 * k / (k + 1), 3)})

@@ -16,6 +16,8 @@
 // launch for grids of more than a few hundred blocks.)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <atomic>
 #include <mutex>
 #include <type_traits>
@@ -254,6 +256,10 @@ static int multidot_launch(hipStream_t st, int ncu, const hpb_tuning_t& tu, int 
     const size_t cus = (size_t)(ncu > 0 ? ncu : 256);
     size_t cap = (size_t)tu.grid_max / (size_t)nch;
     if (cap < cus) cap = cus;   // at least one block per CU per chunk
+    // one chunk: few long-running blocks, as for a single Dot2 (n = 2^24, GB/s at 2 / 1
+    // blocks per CU against the 8 per CU of the table: k = 1: 6250 / 5494 / 5132; k = 2:
+    // 6263 / 5817 / 5553; k = 4: 6510 / 6864 / 6070; k = 8: 6636 / 6146 / 6506)
+    if (nch == 1) cap = std::min(cap, (k == 3 || k == 4 ? 1 : 2) * cus);
     if (cap * (size_t)k > max_pairs) cap = max_pairs / (size_t)k;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
