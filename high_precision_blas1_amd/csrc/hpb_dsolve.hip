@@ -1291,6 +1291,11 @@ private:
         // basis that fits is read back from the MALL by the next pass (128^3 FP32
         // GMRES(30), 248 MB: 0.106 s streamed against 0.098 s)
         const bool nt_basis = (double)(m + 1) * (double)n * sizeof(T) > 256.0 * 1024 * 1024;
+        // multi-dot blocks per CU over all column chunks: 8 from 2^23 rows (256^3 FP64
+        // GMRES(30): 10.99 s, against 11.78 s at 4, 11.10 s at 16), else 4 (128^3:
+        // 8 per CU cost 3-6 %, 2 and 1 per CU 2-9 %); updates: 8 per CU (2 and 1 per CU
+        // were 4-35 % slower, 16 the same)
+        const size_t gd_per_cu = n >= ((size_t)1 << 23) ? 8 : 4;
         const unsigned gupd = (unsigned)std::max<size_t>(1, std::min(gvec, 8 * cus));
         constexpr int KC = 8;
         HPB_HIP(hipStreamSynchronize(st));
@@ -1310,7 +1315,7 @@ private:
                     const int* skip = (dgks && pass == 1) ? &S->skip1 : stop;
                     const int kk = (dgks && pass == 0) ? k + 1 : k;   // + (w, w)
                     const unsigned nch = (unsigned)((kk + KC - 1) / KC);
-                    size_t gd = std::min(gvec, std::max<size_t>(cus, (4 * cus) / nch));
+                    size_t gd = std::min(gvec, std::max<size_t>(cus, (gd_per_cu * cus) / nch));
                     gd = std::max<size_t>(1, std::min(gd, part_pairs / (size_t)kk));
                     auto dot = [&](auto compc, auto ntc) {
                         constexpr bool C = decltype(compc)::value, NTB = decltype(ntc)::value;
