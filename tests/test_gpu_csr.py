@@ -262,6 +262,14 @@ def test_spmv_f32_integer_exact(handle, comp):
     M = sp.random(3000, 3000, density=0.003, format="csr", random_state=3)
     M.data = r.integers(-8, 9, M.nnz).astype(np.float64)
     cases.append(M)
+    # rows of uneven length (20 entries in the first 700 rows, 0-3 after, some empty):
+    # the first 256-row FP32 tiles exceed 2048 entries and read directly, the rest stage
+    lens = np.where(np.arange(3000) < 700, 20, r.integers(0, 4, 3000))
+    U = sp.lil_matrix((3000, 3000))
+    for i, ln in enumerate(lens):
+        cols = r.choice(3000, size=int(ln), replace=False)
+        U[i, cols] = r.integers(-8, 9, int(ln)).astype(np.float64)
+    cases.append(U.tocsr())
     for Mh in cases:
         Mh = Mh.tocsr()
         Mh.sort_indices()
