@@ -74,6 +74,68 @@ def test_round_trip_exact(tmp_path):
     assert np.array_equal(rp, M.indptr) and np.array_equal(ci, M.indices) and va.tobytes() == M.data.tobytes()
 
 
+def _big(rng, n=3000, density=0.03):
+    M = sp.random(n, n, density=density, format="coo", random_state=7)
+    M.data = rng.standard_normal(M.nnz) * np.exp2(rng.integers(-30, 30, M.nnz))
+    return M
+
+
+def test_parallel_read_against_scipy(tmp_path, rng):
+    """A file of several MB is parsed in parallel chunks (threads split at line ends):
+    the CSR equals scipy's reading of the same file, entry for entry."""
+    M = _big(rng)
+    p = tmp_path / "big.mtx"
+    scipy.io.mmwrite(str(p), M, field="real", symmetry="general", precision=17)
+    assert p.stat().st_size > 6 << 20
+    got, rp, ci, va = read(p)
+    ref = scipy.io.mmread(str(p)).tocsr()
+    ref.sort_indices()
+    assert np.array_equal(rp, ref.indptr) and np.array_equal(ci, ref.indices)
+    assert va.tobytes() == ref.data.tobytes()
+
+
+def test_parallel_write_round_trip(tmp_path, rng):
+    """The writer formats row ranges in parallel: a large matrix round-trips bit for
+    bit, and the file equals the one-line-per-entry %.17g text."""
+    M = _big(rng).tocsr()
+    M.sort_indices()
+    p = tmp_path / "w.mtx"
+    write(p, M)
+    got, rp, ci, va = read(p)
+    assert np.array_equal(rp, M.indptr) and np.array_equal(ci, M.indices) and va.tobytes() == M.data.tobytes()
+    lines = p.read_text().splitlines()
+    assert lines[1] == "%d %d %d" % (M.shape[0], M.shape[1], M.nnz) and len(lines) == M.nnz + 2
+    rows = np.repeat(np.arange(M.shape[0]), np.diff(M.indptr))
+    for k in (0, 1, M.nnz // 3, M.nnz // 2, M.nnz - 1):
+        assert lines[k + 2] == "%d %d %.17g" % (rows[k] + 1, M.indices[k] + 1, M.data[k])
+
+
+def test_irregular_lines_read_as_tokens(tmp_path, rng):
+    """Lines the chunk parser does not take (an entry split over two lines, a '+'
+    sign, trailing text after the last entry) send the data through the token parser:
+    the matrix is the one of the regular file."""
+    M = _big(rng)
+    p = tmp_path / "r.mtx"
+    scipy.io.mmwrite(str(p), M, field="real", symmetry="general", precision=17)
+    want = read(p)
+    lines = p.read_text().split("\n")
+    head = next(i for i, ln in enumerate(lines) if ln and not ln.startswith("%")) + 1
+    k = head + len(lines) // 2
+    i, j, v = lines[k].split()
+    variants = [
+        lines[:k] + ["%s %s" % (i, j), v] + lines[k + 1:],
+        lines[:k] + ["%s %s +%s" % (i, j, v.lstrip("+"))] + lines[k + 1:] if not v.startswith("-") else None,
+        lines + ["this is not an entry"],
+    ]
+    for n, ls in enumerate(variants):
+        if ls is None:
+            continue
+        q = tmp_path / ("r%d.mtx" % n)
+        q.write_text("\n".join(ls))
+        got = read(q)
+        assert np.array_equal(got[1], want[1]) and np.array_equal(got[2], want[2]) and got[3].tobytes() == want[3].tobytes(), n
+
+
 def test_comments_and_duplicates_kept(tmp_path):
     p = tmp_path / "d.mtx"
     p.write_text("%%MatrixMarket matrix coordinate real general\n% a comment\n%\n3 3 4\n2 2 1.5\n1 3 2.0\n2 2 -0.5\n"
