@@ -110,6 +110,7 @@ int hpb_destroy(hpb_handle_t h)
     if (h->hres) (void)hipHostFree(h->hres);
     if (h->dflag) (void)hipFree(h->dflag);
     if (h->nrmfix) (void)hipFree(h->nrmfix);
+    if (h->hstage) (void)hipHostFree(h->hstage);
     for (int i = 0; i < 2; ++i)
         if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);

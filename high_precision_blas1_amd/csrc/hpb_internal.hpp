@@ -29,6 +29,8 @@ struct hpb_handle_s {
     void* hres = nullptr;           // pinned host mirror of dres
     int* dflag = nullptr;           // validity flag
     void* nrmfix = nullptr;         // hpb::nrm_fix record of the scaled norms (zero at rest)
+    void* hstage = nullptr;         // pinned host staging of the host-side analyses (grown on demand)
+    size_t hstage_bytes = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     hpb_tuning_t tune_red[2];       // two-input reductions (dot), [0] f64, [1] f32
     hpb_tuning_t tune_red1[2];      // one-input reductions (sum, asum, nrm2)
