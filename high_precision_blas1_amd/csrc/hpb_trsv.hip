@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "hpb_device.hpp"
@@ -224,23 +225,59 @@ int hpb_csr_trsv_analysis(hpb_handle_t h, int uplo, int rows, int64_t nnz, const
     HPB_HIP(hipMemcpyAsync(ci, col_ind, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost, h->stream));
     HPB_HIP(hipStreamSynchronize(h->stream));
     HPB_REQUIRE(rp[0] == 0 && rp[rows] == nnz, HPB_ERR_INVALID, "hpb_csr_trsv_analysis: row_ptr inconsistent with nnz");
-    std::vector<int> level((size_t)rows, 0), diag((size_t)rows, -1);
+    // checks, diagonal positions and the end of each row's lower part (the columns
+    // are sorted, so a row's j < i come first), in parallel over row ranges; the
+    // first failing row in dependency order names the error, as a sequential scan
+    std::vector<int> level((size_t)rows, 0), diag((size_t)rows, -1), lend((size_t)rows);
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned T = (unsigned)std::max<int64_t>(1, std::min<int64_t>({(int64_t)hw, 16, (int64_t)rows / 65536 + 1}));
+        std::vector<int> first_code(T, 0);
+        auto scan = [&](unsigned t) {
+            const int s0 = (int)((int64_t)rows * t / T), s1 = (int)((int64_t)rows * (t + 1) / T);
+            for (int s = s0; s < s1; ++s) {
+                const int i = lower ? s : rows - 1 - s;   // dependency order
+                int code = 0, le = rp[i + 1];
+                for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                    const int j = ci[k];
+                    if (!(j >= 0 && j < rows)) { code = 1; break; }
+                    if (!(k == rp[i] || ci[k - 1] < j)) { code = 2; break; }
+                    if (j >= i && le == rp[i + 1]) le = k;
+                    if (j == i) diag[i] = k;
+                }
+                lend[i] = le;
+                if (!code && !unit && diag[i] < 0) code = 3;
+                if (code) {   // later rows of this range cannot come first
+                    first_code[t] = code;
+                    return;
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < T; ++t) th.emplace_back(scan, t);
+        scan(0);
+        for (auto& x : th) x.join();
+        for (unsigned t = 0; t < T; ++t) {   // ranges in dependency order: the first error wins
+            HPB_REQUIRE(first_code[t] != 1, HPB_ERR_INVALID, "hpb_csr_trsv_analysis: column index out of range");
+            HPB_REQUIRE(first_code[t] != 2, HPB_ERR_INVALID,
+                        "hpb_csr_trsv_analysis: columns must be sorted and unique within a row");
+            HPB_REQUIRE(first_code[t] != 3, HPB_ERR_INVALID,
+                        "hpb_csr_trsv_analysis: a row without a stored diagonal entry (structurally singular)");
+        }
+    }
+    // levels (longest dependency chain): one pass in dependency order over the
+    // dependencies only
     int nlev = 0;
     for (int s = 0; s < rows; ++s) {
-        const int i = lower ? s : rows - 1 - s;   // dependency order
+        const int i = lower ? s : rows - 1 - s;
         int lv = 0;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const int j = ci[k];
-            HPB_REQUIRE(j >= 0 && j < rows, HPB_ERR_INVALID, "hpb_csr_trsv_analysis: column index out of range");
-            HPB_REQUIRE(k == rp[i] || ci[k - 1] < j, HPB_ERR_INVALID,
-                        "hpb_csr_trsv_analysis: columns must be sorted and unique within a row");
-            if (j == i) diag[i] = k;
-            else if (lower ? j < i : j > i) lv = std::max(lv, level[j] + 1);
+        if (lower) {
+            for (int k = rp[i]; k < lend[i]; ++k) lv = std::max(lv, level[ci[k]] + 1);
+        } else {
+            for (int k = lend[i] + (diag[i] >= 0 ? 1 : 0); k < rp[i + 1]; ++k) lv = std::max(lv, level[ci[k]] + 1);
         }
         level[i] = lv;
         nlev = std::max(nlev, lv + 1);
-        HPB_REQUIRE(unit || diag[i] >= 0, HPB_ERR_INVALID,
-                    "hpb_csr_trsv_analysis: a row without a stored diagonal entry (structurally singular)");
     }
     hpb_trsv_s* t = new hpb_trsv_s();
     t->uplo = uplo;
