@@ -363,7 +363,9 @@ int hpb_csr_transpose_f32(hpb_handle_t h, int rows, int cols, int64_t nnz, const
  *           these are the reference's transposed solves L^T x = alpha b (uplo 3 on B)
  *           and U^T x = alpha b (uplo 2 on B), gpu_matrix.h:142-154,232-244.
  * One kernel per level; compensated = 1 evaluates each row's sum with Dot2. x may
- * alias b. */
+ * alias b.  The analysis copies the pattern to the host through a pinned buffer that
+ * the handle keeps, (rows + 1 + nnz) * 4 bytes at the largest pattern analysed, until
+ * hpb_destroy. */
 typedef struct hpb_trsv_s* hpb_trsv_t;
 int hpb_csr_trsv_analysis(hpb_handle_t h, int uplo, int rows, int64_t nnz, const int* row_ptr, const int* col_ind,
                           hpb_trsv_t* out);
