@@ -99,6 +99,35 @@ def test_levels_closed_form_and_errors(handle):
     H.call("hpb_trsv_destroy", t)
 
 
+def test_analysis_first_error_in_dependency_order(handle):
+    """The analysis checks rows in parallel ranges, yet names the error of the first
+    bad row in dependency order (ascending rows for the lower triangle, descending for
+    the upper), as one scan in that order does."""
+    n = 300000
+    T = sp.diags([np.full(n - 1, -1.0), np.full(n, 4.0), np.full(n - 1, -1.0)], [-1, 0, 1], format="csr")
+    rp = T.indptr.astype(np.int32).copy()
+    ci = T.indices.astype(np.int32).copy()
+    lo_bad, hi_bad = 100, 250000
+    a = rp[lo_bad]
+    ci[a], ci[a + 1] = ci[a + 1], ci[a]          # row 100: columns out of order
+    ci[rp[hi_bad + 1] - 1] = n + 5                # row 250000: a column out of range
+    drp = H.DeviceArray.from_numpy(rp, handle)
+    dci = H.DeviceArray.from_numpy(ci, handle)
+    t = ctypes.c_void_p()
+    for uplo, msg in ((0, "sorted"), (2, "sorted"), (1, "out of range"), (3, "out of range")):
+        say("analysis errors", uplo)
+        with pytest.raises(H.HPBError) as e:
+            H.call("hpb_csr_trsv_analysis", handle.ptr, uplo, n, int(rp[-1]), drp.ptr, dci.ptr, ctypes.byref(t))
+        assert msg in str(e.value), (uplo, str(e.value))
+    # the same pattern without the two faults analyses, with 2 levels per 1 row
+    good_ci = H.DeviceArray.from_numpy(T.indices.astype(np.int32), handle)
+    t = _analysis(handle, 0, T, drp, good_ci)
+    nl = ctypes.c_int()
+    H.call("hpb_csr_trsv_levels", t, ctypes.byref(nl))
+    assert nl.value == n
+    H.call("hpb_trsv_destroy", t)
+
+
 def _ilu0(handle, M):
     rp, ci, va, Ms = _dev_csr(handle, M)
     t = _analysis(handle, 0, Ms, rp, ci)
