@@ -77,6 +77,29 @@ int main(int argc, char** argv)
     };
     for (const tcase& c : cases)
         if (int rc = read_case(dir, c.name, c.text)) return rc;
+    // files of several MB, parsed by threads in chunks (big), and the same with one
+    // entry split over two lines, which sends them through the token parser (big_split)
+    {
+        const int n = 150000;
+        std::string big = "%%MatrixMarket matrix coordinate real general\n150000 150000 600000\n", split;
+        char line[96];
+        for (int i = 1; i <= n; ++i)
+            for (int q = 0; q < 4; ++q) {
+                const int j = (int)(((long long)i * (2 * q + 7)) % n) + 1;
+                std::snprintf(line, sizeof line, "%d %d %.17g\n", i, j, (q - 1.5) * 1.0e-3 * i);
+                if (i == n / 2 && q == 1) {   // "i j\nv\n" in the copy
+                    std::string l(line);
+                    const size_t sp = l.rfind(' ');
+                    split += l.substr(0, sp) + "\n" + l.substr(sp + 1);
+                } else {
+                    split += line;
+                }
+                big += line;
+            }
+        split = big.substr(0, big.find('\n', big.find('\n') + 1) + 1) + split;
+        if (int rc = read_case(dir, "big", big.c_str())) return rc;
+        if (int rc = read_case(dir, "big_split", split.c_str())) return rc;
+    }
     int rows = 0, cols = 0;
     int64_t nnz = 0;
     int *rp = nullptr, *ci = nullptr;

@@ -76,10 +76,11 @@ def test_matrix_market_sanitized(tmp_path):
     assert "runtime error" not in p.stderr and "AddressSanitizer" not in p.stderr, p.stderr[-3000:]
     res = _parse(p.stdout)
     ok = {"general": (3, 5), "symmetric": (4, 6), "skew": (3, 4), "integer": (2, 3), "pattern": (3, 3),
-          "duplicates": (2, 4), "empty": (5, 0)}
+          "duplicates": (2, 4), "empty": (5, 0), "big": (150000, 600000), "big_split": (150000, 600000)}
     for name, (rows, nnz) in ok.items():
         st, r, k, _ = res[name]
         assert (int(st), int(r), int(k)) == (0, rows, nnz), name
+    assert res["big"][3] == res["big_split"][3]   # the same matrix through either parser
     for name in ("bad_banner", "bad_index", "zero_index", "short", "array", "complex", "garbage_value", "no_size",
                  "huge_size"):
         assert int(res[name][0]) != 0, name
