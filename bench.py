@@ -302,6 +302,7 @@ def cfg_spmv(H, h, log):
                            ctypes.c_double(0.0), ys.ptr), 50, 20)
         res[name + "_gbs"] = round(nbytes / ms / 1e6, 1)
         res[name + "_ms"] = round(ms, 4)
+        res[name + "_hbm_frac"] = round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)
     A.free()
     xs.free()
     ys.free()
