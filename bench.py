@@ -681,7 +681,7 @@ def cfg_next(H, h, log):
     H.call("hpb_random_uniform_f64", h.ptr, 2 * nc, 11, 0, ctypes.c_double(-1.0), ctypes.c_double(1.0), zx.ptr)
     H.call("hpb_random_uniform_f64", h.ptr, 2 * nc, 11, 1, ctypes.c_double(-1.0), ctypes.c_double(1.0), zy.ptr)
     d6 = H.DeviceArray(8, np.float64, h)
-    ms = timed(h, L.hpb_zdot2_f64_async, (h.ptr, nc, zx.ptr, zy.ptr, d6.ptr), 20, 3)
+    ms = timed(h, L.hpb_zdot2_f64_async, (h.ptr, nc, zx.ptr, zy.ptr, d6.ptr), 50, 20)
     out["complex_dot2"] = {"n_complex": nc, "ms": round(ms, 4), "gbs_algorithmic": round(32.0 * nc / ms / 1e6, 1)}
     for a in (zx, zy, d6):
         a.free()
