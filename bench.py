@@ -239,7 +239,7 @@ def cfg_cond_sweep(H, h, n, log):
         row = {"cond_target": cond, "cond_achieved": float(dd[2]), "dot2_gbs": round(16.0 * n / ms / 1e6, 1),
                "dot2_rel_err": rel_err(g[2], ex), "naive_rel_err": rel_err(out[2], ex)}
         # device exact dot (long accumulator) against the generator's tracked value
-        ms_x = timed(h, L.hpb_exact_dot_f64_async, (h.ptr, n, x.ptr, y.ptr, d.ptr), 5, 1)
+        ms_x = timed(h, L.hpb_exact_dot_f64_async, (h.ptr, n, x.ptr, y.ptr, d.ptr), 20, 5)
         e3 = d.numpy()[:3]
         hi = float(ex)
         row["exact_dot_gbs"] = round(16.0 * n / ms_x / 1e6, 1)
