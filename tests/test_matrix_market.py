@@ -110,6 +110,22 @@ def test_parallel_write_round_trip(tmp_path, rng):
         assert lines[k + 2] == "%d %d %.17g" % (rows[k] + 1, M.indices[k] + 1, M.data[k])
 
 
+def test_parallel_write_skewed_rows(tmp_path, rng):
+    """Row ranges of about equal entry counts when the entries sit in a few rows: one
+    row of 300 000 entries among empty rows, written in parallel and read back."""
+    n = 400000
+    cols = np.sort(rng.choice(n, 300000, replace=False)).astype(np.int32)
+    rows = np.full(cols.size, 123456, dtype=np.int32)
+    rows[:10] = 7
+    rows[-5:] = n - 1
+    M = sp.csr_matrix((rng.standard_normal(cols.size), (rows, cols)), shape=(n, n))
+    M.sort_indices()
+    p = tmp_path / "skew.mtx"
+    write(p, M)
+    got, rp, ci, va = read(p)
+    assert np.array_equal(rp, M.indptr) and np.array_equal(ci, M.indices) and va.tobytes() == M.data.tobytes()
+
+
 def test_irregular_lines_read_as_tokens(tmp_path, rng):
     """Lines the chunk parser does not take (an entry split over two lines, a '+'
     sign, trailing text after the last entry) send the data through the token parser:
